@@ -1,0 +1,187 @@
+/*
+ * libmcint: libcint's two-electron Rys-quadrature integrals on one AMD
+ * MI355X (gfx950).  C ABI.
+ *
+ * The data model is libcint's (include/cint.h.in:26-67 of libcint 6.1.1):
+ * atm[natm*6], bas[nbas*8] (FINT = int), env[] doubles with user data from
+ * slot 20, contraction coefficients already multiplied by CINTgto_norm.
+ * Outputs are column-major (di,dj,dk,dl[,comp]) blocks exactly as libcint
+ * writes them.
+ *
+ * Three groups of entry points:
+ *   1. libcint-named per-call functions (drop-in; each call runs on the GPU
+ *      and synchronises, so they are correct but launch-bound);
+ *   2. *_batch functions taking a list of shell tuples on the host: the
+ *      drop-in for a loop over the per-call functions;
+ *   3. mcint_* functions with device-resident outputs, reusable plans and
+ *      HIP streams: the throughput path.
+ * All functions return status codes instead of calling exit().
+ */
+#ifndef MCINT_H
+#define MCINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef FINT
+#define FINT int
+#endif
+#ifndef CACHE_SIZE_T
+#define CACHE_SIZE_T FINT
+#endif
+
+#if !defined HAVE_DEFINED_CINTOPT_H
+#define HAVE_DEFINED_CINTOPT_H
+/* Same public layout as libcint's PairData / CINTOpt (include/cint.h.in:140-152).
+ * libmcint allocates a larger private object behind this header; device
+ * data hangs off it.  Fields other than nbas are left NULL. */
+typedef struct {
+        double rij[3];
+        double eij;
+        double cceij;
+} PairData;
+typedef struct {
+        FINT **index_xyz_array;
+        FINT **non0ctr;
+        FINT **sortedidx;
+        FINT nbas;
+        double **log_max_coeff;
+        PairData **pairdata;
+} CINTOpt;
+#endif
+
+/* Status codes */
+#define MCINT_OK 0
+#define MCINT_ERR_ARG (-1)       /* invalid argument (shell index, l, null pointer) */
+#define MCINT_ERR_UNSUPPORTED (-2) /* class or operator outside the device tables */
+#define MCINT_ERR_HIP (-3)       /* a HIP runtime call failed */
+#define MCINT_ERR_NOMEM (-4)     /* device or host allocation failed */
+
+/* Integral kinds for the plan API */
+#define MCINT_INT2E_SPH 0
+#define MCINT_INT2E_CART 1
+#define MCINT_INT2E_IP1_SPH 2
+#define MCINT_INT2E_IP1_CART 3
+#define MCINT_INT3C2E_SPH 4
+#define MCINT_INT3C2E_CART 5
+
+/* ---- 1. libcint-named per-call entry points --------------------------
+ * Signature of libcint's CINTIntegralFunction (include/cint_funcs.h:10-17).
+ * Return: nonzero iff any primitive survived screening; the block is
+ * zero-filled otherwise (src/cint2e.c:857-869).  out == NULL returns a
+ * positive size hint (as libcint returns the cache size, src/cint2e.c:801-816).
+ * On a device error the functions return 0 and mcint_last_error() says why. */
+CACHE_SIZE_T int2e_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                       FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                       /* replaces src/cint2e.c:1186-1193 */
+CACHE_SIZE_T int2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                        FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                        /* replaces src/cint2e.c:1202-1210 */
+CACHE_SIZE_T int2e_ip1_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                           FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                           /* replaces src/autocode/grad2.c:81-88 */
+CACHE_SIZE_T int2e_ip1_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                            FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                            /* replaces src/autocode/grad2.c:73-80 */
+CACHE_SIZE_T int3c2e_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                         FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                         /* replaces src/cint3c2e.c:693-701 */
+CACHE_SIZE_T int3c2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                          FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+                          /* replaces src/cint3c2e.c:709-717 */
+
+/* Optimizers (src/cint2e.c:1195-1200, src/autocode/grad2.c:69-72,
+ * src/cint3c2e.c:703-707) and their destructors (src/optimizer.c:40-72). */
+void int2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void int2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void int3c2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void CINTdel_optimizer(CINTOpt **opt);
+void CINTdel_2e_optimizer(CINTOpt **opt);
+
+/* Legacy names (src/misc.h:34-83, src/c2f.h:6-64) */
+FINT cint2e_sph(double *out, FINT *shls, FINT *atm, FINT natm, FINT *bas, FINT nbas,
+                double *env, CINTOpt *opt);
+FINT cint2e_cart(double *out, FINT *shls, FINT *atm, FINT natm, FINT *bas, FINT nbas,
+                 double *env, CINTOpt *opt);
+FINT cint2e_ip1_sph(double *out, FINT *shls, FINT *atm, FINT natm, FINT *bas, FINT nbas,
+                    double *env, CINTOpt *opt);
+void cint2e_sph_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+
+/* Basis helpers (src/misc.c:72-93, src/cint_bas.c:36-160) */
+double CINTgto_norm(FINT n, double a);
+FINT CINTcgto_spheric(const FINT bas_id, const FINT *bas);
+FINT CINTcgto_cart(const FINT bas_id, const FINT *bas);
+FINT CINTtot_cgto_spheric(const FINT *bas, const FINT nbas);
+FINT CINTtot_cgto_cart(const FINT *bas, const FINT nbas);
+void CINTshells_spheric_offset(FINT ao_loc[], const FINT *bas, const FINT nbas);
+void CINTshells_cart_offset(FINT ao_loc[], const FINT *bas, const FINT nbas);
+
+/* Rys roots u = t^2/(1-t^2) and weights of exp(-x t^2) on [0,1], evaluated
+ * on the device (src/rys_roots.c:57-122 computes the same rule).
+ * nroots <= 9.  Returns MCINT_OK or an error code. */
+int CINTrys_roots(int nroots, double x, double *u, double *w);
+int mcint_rys_roots_batch(int nroots, const double *x, size_t n, double *u, double *w);
+
+/* ---- 2. host-pointer batches -------------------------------------------
+ * out + out_offsets[q] receives quartet q, byte-identical in layout to
+ * int2e_sph(out + out_offsets[q], NULL, shls + 4*q, ...).  out_offsets ==
+ * NULL packs the blocks in list order.  not0 (nullable) receives each
+ * quartet's return value.  hip_stream may be NULL.  Returns MCINT_OK or an
+ * error code.  For int3c2e the list stride is still 4 (the 4th entry is
+ * ignored). */
+int int2e_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                    const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                    const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                     const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                     const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_ip1_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                        const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                        const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int3c2e_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                      const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                      const CINTOpt *opt, FINT *not0, void *hip_stream);
+
+/* ---- 3. device-resident plans ------------------------------------------- */
+typedef struct mcint_basis mcint_basis;
+typedef struct mcint_plan mcint_plan;
+
+/* Upload atm/bas/env (host pointers) to the current device. */
+int mcint_basis_create(mcint_basis **b, const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                       const double *env);
+void mcint_basis_destroy(mcint_basis *b);
+
+/* Build a plan for a host list of shell tuples (stride 4): canonical class
+ * buckets, primitive-pair data, launch table.  out_offsets (host, nullable
+ * = packed) in doubles.  *out_size receives the number of doubles the
+ * output must hold. */
+int mcint_plan_create(mcint_plan **p, mcint_basis *b, int intor, const FINT *shls, size_t nq,
+                      const int64_t *out_offsets, int64_t *out_size);
+/* Run the plan: out_dev and not0_dev (nullable) are device pointers. The
+ * first call on given (out, not0, stream) captures a HIP graph that later
+ * calls replay.  Asynchronous on hip_stream. */
+int mcint_plan_execute(mcint_plan *p, double *out_dev, FINT *not0_dev, void *hip_stream);
+/* Same, launching the kernels directly (no graph). */
+int mcint_plan_execute_direct(mcint_plan *p, double *out_dev, FINT *not0_dev, void *hip_stream);
+void mcint_plan_destroy(mcint_plan *p);
+
+/* Plan statistics: n[0] tasks, n[1] buckets, n[2] tasks on fast kernels,
+ * n[3] tasks on the generic kernel, n[4] primitive quartets before
+ * screening, n[5] output doubles. */
+int mcint_plan_stats(const mcint_plan *p, int64_t *n);
+
+/* Schwarz bounds q[i*nbas+j] = sqrt(max |(ij|ij)|) over the spherical
+ * block of every shell pair among the first nshell shells (nshell <= 0:
+ * all).  q is a host array of nbas*nbas doubles. */
+int mcint_schwarz(mcint_basis *b, int nshell, double *q);
+
+const char *mcint_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCINT_H */

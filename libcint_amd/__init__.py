@@ -1,0 +1,240 @@
+"""libcint_amd: libcint's two-electron Rys integrals on an AMD MI355X.
+
+Python mirror of the C ABI in include/mcint.h (ctypes).  The product is the
+shared library libcint_amd/libmcint.so, built by ``python -m
+libcint_amd.build``; importing this package never falls back to a CPU
+implementation -- if the library is missing, ``lib()`` raises.
+
+Names follow libcint: ``intor('int2e_sph', shls, basis)`` is one call of
+libcint's ``int2e_sph`` (src/cint2e.c:1186-1193); ``batch`` is the drop-in for
+a loop over such calls; ``Plan`` is the device-resident throughput path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmcint.so")
+
+INT2E_SPH, INT2E_CART, INT2E_IP1_SPH, INT2E_IP1_CART, INT3C2E_SPH, INT3C2E_CART = range(6)
+INTOR_CODES = {
+    "int2e_sph": INT2E_SPH,
+    "int2e_cart": INT2E_CART,
+    "int2e_ip1_sph": INT2E_IP1_SPH,
+    "int2e_ip1_cart": INT2E_IP1_CART,
+    "int3c2e_sph": INT3C2E_SPH,
+    "int3c2e_cart": INT3C2E_CART,
+}
+
+_lib = None
+
+
+class McintError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmcint.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise McintError(f"{LIB_PATH} is missing: run `python -m libcint_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
+                 "int3c2e_cart"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, vp, dp]
+    for name in ("int2e_sph_batch", "int2e_cart_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [dp, ctypes.POINTER(ctypes.c_size_t), ip, ctypes.c_size_t, ip, ctypes.c_int,
+                      ip, ctypes.c_int, dp, vp, ip, vp]
+    L.mcint_basis_create.restype = ctypes.c_int
+    L.mcint_basis_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_int, ip, ctypes.c_int, dp]
+    L.mcint_basis_destroy.argtypes = [vp]
+    L.mcint_plan_create.restype = ctypes.c_int
+    L.mcint_plan_create.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ip, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_execute.restype = ctypes.c_int
+    L.mcint_plan_execute.argtypes = [vp, vp, vp, vp]
+    L.mcint_plan_execute_direct.restype = ctypes.c_int
+    L.mcint_plan_execute_direct.argtypes = [vp, vp, vp, vp]
+    L.mcint_plan_destroy.argtypes = [vp]
+    L.mcint_plan_stats.restype = ctypes.c_int
+    L.mcint_plan_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_schwarz.restype = ctypes.c_int
+    L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
+    L.mcint_last_error.restype = ctypes.c_char_p
+    L.CINTrys_roots.restype = ctypes.c_int
+    L.CINTrys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
+    L.mcint_rys_roots_batch.restype = ctypes.c_int
+    L.mcint_rys_roots_batch.argtypes = [ctypes.c_int, dp, ctypes.c_size_t, dp, dp]
+    L.CINTgto_norm.restype = ctypes.c_double
+    L.CINTgto_norm.argtypes = [ctypes.c_int, ctypes.c_double]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().mcint_last_error().decode()
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise McintError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Basis:
+    """atm/bas/env arrays (host), in libcint's layout."""
+
+    def __init__(self, atm, bas, env):
+        self.atm = np.ascontiguousarray(atm, dtype=np.int32)
+        self.bas = np.ascontiguousarray(bas, dtype=np.int32)
+        self.env = np.ascontiguousarray(env, dtype=np.float64)
+        self.natm = int(self.atm.shape[0])
+        self.nbas = int(self.bas.shape[0])
+
+    def shell_dim(self, ish: int, sph: bool = True) -> int:
+        l = int(self.bas[ish, 1])
+        return (2 * l + 1 if sph else (l + 1) * (l + 2) // 2) * int(self.bas[ish, 3])
+
+    def block_sizes(self, name: str, shls: np.ndarray) -> np.ndarray:
+        sph = not name.endswith("cart")
+        ncomp = 3 if "ip1" in name else 1
+        naxes = 3 if name.startswith("int3c2e") else 4
+        shls = np.asarray(shls).reshape(-1, 4)
+        size = np.full(shls.shape[0], ncomp, dtype=np.int64)
+        for a in range(naxes):
+            l = self.bas[shls[:, a], 1].astype(np.int64)
+            nc = self.bas[shls[:, a], 3].astype(np.int64)
+            size *= ((2 * l + 1) if sph else (l + 1) * (l + 2) // 2) * nc
+        return size
+
+
+def intor(name: str, shls, basis: Basis, dims=None):
+    """One libcint-style call; returns (Fortran-ordered array, return value)."""
+    L = lib()
+    sph = not name.endswith("cart")
+    ncomp = 3 if "ip1" in name else 1
+    naxes = 3 if name.startswith("int3c2e") else 4
+    d = [basis.shell_dim(int(shls[a]), sph) for a in range(naxes)]
+    out = np.zeros(int(np.prod(d)) * ncomp)
+    sh = np.asarray(list(shls) + [0] * (4 - len(shls)), dtype=np.int32)
+    rc = getattr(L, name)(_dp(out), None, _ip(sh), _ip(basis.atm), basis.natm, _ip(basis.bas),
+                          basis.nbas, _dp(basis.env), None, None)
+    shape = tuple(d) + ((ncomp,) if ncomp > 1 else ())
+    return out.reshape(shape, order="F"), rc
+
+
+def batch(name: str, shls, basis: Basis):
+    """Host batch: packed outputs of a quartet list; returns (out, offsets, not0)."""
+    L = lib()
+    shls = np.ascontiguousarray(shls, dtype=np.int32).reshape(-1, 4)
+    nq = shls.shape[0]
+    sizes = basis.block_sizes(name, shls)
+    off = np.zeros(nq + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    out = np.zeros(max(int(off[-1]), 1))
+    not0 = np.zeros(nq, dtype=np.int32)
+    f = getattr(L, name + "_batch")
+    rc = f(_dp(out), None, _ip(shls), nq, _ip(basis.atm), basis.natm, _ip(basis.bas), basis.nbas,
+           _dp(basis.env), None, _ip(not0), None)
+    _check(rc, name + "_batch")
+    return out[: int(off[-1])], off, not0
+
+
+def rys_roots(nroots: int, x):
+    """Device Rys roots/weights for an array of x: returns (u[n, nroots], w[n, nroots])."""
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    u = np.zeros((x.size, nroots))
+    w = np.zeros((x.size, nroots))
+    _check(lib().mcint_rys_roots_batch(nroots, _dp(x), x.size, _dp(u), _dp(w)),
+           "mcint_rys_roots_batch")
+    return u, w
+
+
+class DeviceBasis:
+    """mcint_basis: atm/bas/env uploaded once to the current device."""
+
+    def __init__(self, basis: Basis):
+        self.basis = basis
+        h = ctypes.c_void_p()
+        _check(lib().mcint_basis_create(ctypes.byref(h), _ip(basis.atm), basis.natm,
+                                        _ip(basis.bas), basis.nbas, _dp(basis.env)),
+               "mcint_basis_create")
+        self.handle = h
+
+    def schwarz(self, nshell: int = 0) -> np.ndarray:
+        q = np.zeros((self.basis.nbas, self.basis.nbas))
+        _check(lib().mcint_schwarz(self.handle, nshell, _dp(q)), "mcint_schwarz")
+        return q
+
+    def close(self):
+        if self.handle:
+            lib().mcint_basis_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """mcint_plan over a host list of shell tuples; outputs live on the device."""
+
+    def __init__(self, dbasis: DeviceBasis, name: str, shls, out_offsets=None):
+        self.dbasis = dbasis
+        self.name = name
+        shls = np.ascontiguousarray(shls, dtype=np.int32).reshape(-1, 4)
+        self.nq = shls.shape[0]
+        h = ctypes.c_void_p()
+        osz = ctypes.c_int64()
+        offp = None
+        if out_offsets is not None:
+            self._off = np.ascontiguousarray(out_offsets, dtype=np.int64)
+            offp = self._off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        _check(lib().mcint_plan_create(ctypes.byref(h), dbasis.handle, INTOR_CODES[name], _ip(shls),
+                                       self.nq, offp, ctypes.byref(osz)), "mcint_plan_create")
+        self.handle = h
+        self.out_size = int(osz.value)
+
+    def stats(self) -> dict:
+        n = (ctypes.c_int64 * 6)()
+        _check(lib().mcint_plan_stats(self.handle, n), "mcint_plan_stats")
+        return dict(tasks=n[0], buckets=n[1], fast_tasks=n[2], generic_tasks=n[3],
+                    prim_quartets=n[4], out_size=n[5])
+
+    def execute(self, out_ptr: int, not0_ptr: int = 0, stream_ptr: int = 0, graph: bool = True):
+        """Run on device pointers (e.g. torch tensor .data_ptr(), stream.cuda_stream)."""
+        f = lib().mcint_plan_execute if graph else lib().mcint_plan_execute_direct
+        _check(f(self.handle, ctypes.c_void_p(out_ptr), ctypes.c_void_p(not0_ptr or None),
+                 ctypes.c_void_p(stream_ptr or None)), "mcint_plan_execute")
+
+    def close(self):
+        if self.handle:
+            lib().mcint_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

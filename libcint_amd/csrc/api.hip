@@ -1,0 +1,901 @@
+// Host driver and C ABI of libmcint.
+//
+// Flow of a batch (the GPU counterpart of looping libcint's int2e_* calls,
+// examples/time_c2h6.c:719-745):
+//   1. mcint_basis_create uploads atm/bas/env and expands generally
+//      contracted shells into virtual shells;
+//   2. mcint_plan_create canonicalises every shell tuple (permutational
+//      symmetry of (ij|kl)), sorts the tuples into buckets of equal
+//      (class, permutation, team size), and builds the primitive-pair data
+//      of every pair the tuples touch (the role of int2e_optimizer,
+//      src/optimizer.c:344-425);
+//   3. mcint_plan_execute launches one kernel per bucket, captured once into
+//      a HIP graph and replayed afterwards.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mcint.h"
+#include "eri_launch.h"
+
+using namespace mcint;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const std::string& s) { g_err = s; }
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_err(std::string(#call) + " failed: " + hipGetErrorString(e_));               \
+      return MCINT_ERR_HIP;                                                            \
+    }                                                                                  \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return hipSuccess;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int intor_kind(int intor) {
+  switch (intor) {
+    case MCINT_INT2E_SPH:
+    case MCINT_INT2E_CART: return KIND_2E;
+    case MCINT_INT2E_IP1_SPH:
+    case MCINT_INT2E_IP1_CART: return KIND_2E_IP1;
+    case MCINT_INT3C2E_SPH:
+    case MCINT_INT3C2E_CART: return KIND_3C2E;
+    default: return -1;
+  }
+}
+
+int intor_sph(int intor) {
+  return intor == MCINT_INT2E_SPH || intor == MCINT_INT2E_IP1_SPH || intor == MCINT_INT3C2E_SPH;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Basis
+// ---------------------------------------------------------------------------
+struct mcint_basis {
+  int natm = 0, nbas = 0, nv = 0;
+  std::vector<int> atm, bas;
+  std::vector<double> env;
+  std::vector<VShell> vsh;
+  std::vector<int> vfirst;  // first virtual shell of each real shell
+  VShell* vsh_d = nullptr;
+  double* env_d = nullptr;
+  double expc4 = EXPCUTOFF, expc3 = EXPCUTOFF, omega = 0;
+  int device = 0;
+};
+
+extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm, const FINT* bas,
+                                  FINT nbas, const double* env) {
+  if (!out || !atm || !bas || !env || natm <= 0 || nbas <= 0) {
+    set_err("mcint_basis_create: null pointer or empty basis");
+    return MCINT_ERR_ARG;
+  }
+  *out = nullptr;
+  auto* b = new mcint_basis();
+  b->natm = natm;
+  b->nbas = nbas;
+  b->atm.assign(atm, atm + (size_t)natm * ATM_SLOTS);
+  b->bas.assign(bas, bas + (size_t)nbas * BAS_SLOTS);
+  // env has no length argument in libcint's ABI: take the largest slot any
+  // atom or shell refers to.
+  size_t envlen = 20;
+  for (int a = 0; a < natm; ++a) envlen = std::max(envlen, (size_t)atm[a * ATM_SLOTS + PTR_COORD] + 3);
+  for (int s = 0; s < nbas; ++s) {
+    const int* bs = bas + (size_t)s * BAS_SLOTS;
+    if (bs[ATOM_OF] < 0 || bs[ATOM_OF] >= natm || bs[ANG_OF] < 0 || bs[ANG_OF] > LMAX ||
+        bs[NPRIM_OF] <= 0 || bs[NCTR_OF] <= 0 || bs[PTR_EXP] < 0 || bs[PTR_COEFF] < 0) {
+      delete b;
+      char msg[160];
+      snprintf(msg, sizeof msg, "mcint_basis_create: shell %d invalid (atom/l/nprim/nctr); l <= %d",
+               s, LMAX);
+      set_err(msg);
+      return MCINT_ERR_ARG;
+    }
+    envlen = std::max(envlen, (size_t)bs[PTR_EXP] + bs[NPRIM_OF]);
+    envlen = std::max(envlen, (size_t)bs[PTR_COEFF] + (size_t)bs[NPRIM_OF] * bs[NCTR_OF]);
+  }
+  b->env.assign(env, env + envlen);
+  if (env[0] != 0) {
+    b->expc4 = std::max(MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]) + 1;  // src/g2e.c:57-62
+    b->expc3 = std::max(MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]);      // src/g3c2e.c:54-58
+  }
+  b->omega = env[PTR_RANGE_OMEGA];
+  for (int s = 0; s < nbas; ++s) {
+    const int* bs = bas + (size_t)s * BAS_SLOTS;
+    b->vfirst.push_back((int)b->vsh.size());
+    const int l = bs[ANG_OF];
+    for (int c = 0; c < bs[NCTR_OF]; ++c) {
+      VShell v{};
+      const double* r = env + atm[bs[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+      v.r[0] = r[0];
+      v.r[1] = r[1];
+      v.r[2] = r[2];
+      v.l = l;
+      v.nprim = bs[NPRIM_OF];
+      v.pexp = bs[PTR_EXP];
+      v.pcoef = bs[PTR_COEFF] + c * bs[NPRIM_OF];
+      v.pcoef0 = bs[PTR_COEFF];
+      v.nctr = bs[NCTR_OF];
+      v.dsph = nsph(l) * bs[NCTR_OF];
+      v.dcart = ncart(l) * bs[NCTR_OF];
+      v.shell = s;
+      b->vsh.push_back(v);
+    }
+  }
+  b->nv = (int)b->vsh.size();
+  if (b->nv > 32767) {
+    delete b;
+    set_err("mcint_basis_create: more than 32767 contracted shell columns");
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = dalloc(&b->vsh_d, b->vsh.size());
+  if (e == hipSuccess) e = dalloc(&b->env_d, b->env.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->vsh_d, b->vsh.data(), b->vsh.size() * sizeof(VShell), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->env_d, b->env.data(), b->env.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_basis_create: ") + hipGetErrorString(e));
+    dfree(b->vsh_d);
+    dfree(b->env_d);
+    delete b;
+    return MCINT_ERR_HIP;
+  }
+  *out = b;
+  return MCINT_OK;
+}
+
+extern "C" void mcint_basis_destroy(mcint_basis* b) {
+  if (!b) return;
+  dfree(b->vsh_d);
+  dfree(b->env_d);
+  delete b;
+}
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Bucket {
+  int la, lb, lc, ld, perm, team;
+  bool fast;
+  int64_t t0, t1;
+  int64_t sstride;  // generic scratch doubles per task
+};
+
+// Largest number of generic tasks launched at once (bounds the scratch).
+constexpr int64_t GEN_SCRATCH_DOUBLES = 32ll << 20;
+
+int pick_team(int64_t nprim_quartets) {
+  int t = 1;
+  while (t < fast_team_lanes_max() && (int64_t)t * 8 < nprim_quartets) t *= 2;
+  return t;
+}
+
+}  // namespace
+
+struct mcint_plan {
+  mcint_basis* b = nullptr;
+  int intor = 0, kind = 0, sph = 1, ncomp = 1;
+  int64_t nq = 0, ntasks = 0, out_size = 0;
+  std::vector<Bucket> buckets;
+  Task* tasks_d = nullptr;
+  int* task_id_d = nullptr;
+  int64_t* pair_off_d = nullptr;
+  double* pair_ld_d = nullptr;
+  PairPrim* pp_d = nullptr;
+  double* scratch_d = nullptr;
+  int64_t scratch_size = 0;
+  int64_t stats[6] = {0, 0, 0, 0, 0, 0};
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double* g_out = nullptr;
+  FINT* g_not0 = nullptr;
+  hipStream_t g_stream = nullptr;
+};
+
+static void plan_free(mcint_plan* p) {
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  dfree(p->tasks_d);
+  dfree(p->task_id_d);
+  dfree(p->pair_off_d);
+  dfree(p->pair_ld_d);
+  dfree(p->pp_d);
+  dfree(p->scratch_d);
+  delete p;
+}
+
+extern "C" void mcint_plan_destroy(mcint_plan* p) { plan_free(p); }
+
+extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
+                                 size_t nq, const int64_t* out_offsets, int64_t* out_size) {
+  if (!out || !b || (!shls && nq)) {
+    set_err("mcint_plan_create: null argument");
+    return MCINT_ERR_ARG;
+  }
+  *out = nullptr;
+  const int kind = intor_kind(intor);
+  if (kind < 0) {
+    set_err("mcint_plan_create: unknown integral kind");
+    return MCINT_ERR_ARG;
+  }
+  if (kind != KIND_3C2E && b->omega < 0) {
+    set_err("mcint_plan_create: short-range (omega < 0) 4-centre integrals are not supported");
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  if (kind == KIND_3C2E && b->omega != 0) {
+    set_err("mcint_plan_create: range-separated 3-centre integrals are not supported");
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  const int sph = intor_sph(intor);
+  const int ncomp = kind == KIND_2E_IP1 ? 3 : 1;
+  const int naxes = kind == KIND_3C2E ? 3 : 4;
+  const int* bas = b->bas.data();
+  const int nbas = b->nbas;
+
+  auto* p = new mcint_plan();
+  p->b = b;
+  p->intor = intor;
+  p->kind = kind;
+  p->sph = sph;
+  p->ncomp = ncomp;
+  p->nq = (int64_t)nq;
+
+  // ---- validate, offsets, canonical order, bucket keys -------------------
+  std::vector<int64_t> qoff(nq + 1, 0);
+  std::vector<uint32_t> key;  // per task
+  std::vector<Task> tasks;
+  std::vector<int> task_q;
+  tasks.reserve(nq);
+  key.reserve(nq);
+  task_q.reserve(nq);
+  int64_t running = 0;
+  int64_t nprimq_total = 0;
+  for (size_t q = 0; q < nq; ++q) {
+    const FINT* s = shls + 4 * q;
+    int l[4] = {0, 0, 0, 0}, w[4] = {1, 1, 1, 1}, nc[4] = {1, 1, 1, 1}, np[4] = {1, 1, 1, 1};
+    for (int a = 0; a < naxes; ++a) {
+      if (s[a] < 0 || s[a] >= nbas) {
+        plan_free(p);
+        char msg[128];
+        snprintf(msg, sizeof msg, "mcint_plan_create: quartet %zu has shell index %d out of range",
+                 q, (int)s[a]);
+        set_err(msg);
+        return MCINT_ERR_ARG;
+      }
+      const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
+      l[a] = bs[ANG_OF];
+      nc[a] = bs[NCTR_OF];
+      np[a] = bs[NPRIM_OF];
+      w[a] = sph ? nsph(l[a]) : ncart(l[a]);
+    }
+    const int lsum = l[0] + l[1] + l[2] + l[3] + (kind == KIND_2E_IP1 ? 1 : 0);
+    if (lsum / 2 + 1 > NRYS_MAX) {
+      plan_free(p);
+      char msg[160];
+      snprintf(msg, sizeof msg,
+               "mcint_plan_create: quartet %zu needs %d Rys roots; the device tables hold %d",
+               q, lsum / 2 + 1, NRYS_MAX);
+      set_err(msg);
+      return MCINT_ERR_UNSUPPORTED;
+    }
+    int64_t so[5];
+    so[0] = 1;
+    for (int o = 0; o < 4; ++o) so[o + 1] = so[o] * (int64_t)w[o] * nc[o];
+    const int64_t blk = so[4] * ncomp;
+    qoff[q] = out_offsets ? out_offsets[q] : running;
+    running += blk;
+
+    // canonical axis order L[A] = original axis
+    int L[4] = {0, 1, 2, 3};
+    if (kind == KIND_2E) {
+      if (l[L[0]] < l[L[1]]) std::swap(L[0], L[1]);
+      if (l[L[2]] < l[L[3]]) std::swap(L[2], L[3]);
+      if (l[L[2]] > l[L[0]] || (l[L[2]] == l[L[0]] && l[L[3]] > l[L[1]])) {
+        int t0 = L[0], t1 = L[1];
+        L[0] = L[2];
+        L[1] = L[3];
+        L[2] = t0;
+        L[3] = t1;
+      }
+    } else if (kind == KIND_2E_IP1) {
+      if (l[2] < l[3]) std::swap(L[2], L[3]);
+    } else {
+      if (l[0] < l[1]) std::swap(L[0], L[1]);
+    }
+    const int perm = L[0] | (L[1] << 2) | (L[2] << 4) | (L[3] << 6);
+    const int la = l[L[0]], lb = l[L[1]], lc = l[L[2]], ld = kind == KIND_3C2E ? 0 : l[L[3]];
+    const int64_t nprimq = (int64_t)np[0] * np[1] * np[2] * (naxes == 4 ? np[3] : 1);
+    nprimq_total += nprimq;
+    const bool fast = fast_kernel_available(kind, sph, la, lb, lc, ld);
+    const int team = fast ? pick_team(nprimq) : 0;
+    int tl = 0;
+    while ((1 << tl) < team) ++tl;
+    const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |
+                       ((uint32_t)ld << 9) | ((uint32_t)perm << 12) | ((uint32_t)tl << 20) |
+                       ((uint32_t)fast << 24);
+    // virtual expansion over contraction columns of the original axes
+    int cidx[4] = {0, 0, 0, 0};
+    for (;;) {
+      Task t;
+      int64_t o = qoff[q];
+      for (int a = 0; a < 4; ++a) {
+        if (a < naxes) o += (int64_t)cidx[a] * w[a] * so[a];
+      }
+      for (int A = 0; A < 4; ++A) {
+        if (A < naxes) {
+          const int orig = L[A];
+          t.sh[A] = (int16_t)(b->vfirst[s[orig]] + cidx[orig]);
+        } else {
+          t.sh[A] = t.sh[naxes - 1];
+        }
+      }
+      t.out = o;
+      tasks.push_back(t);
+      key.push_back(k);
+      task_q.push_back((int)q);
+      int a = 0;
+      for (; a < naxes; ++a) {
+        if (++cidx[a] < nc[a]) break;
+        cidx[a] = 0;
+      }
+      if (a == naxes) break;
+    }
+  }
+  p->out_size = out_offsets ? 0 : running;
+  if (out_offsets) {
+    int64_t mx = 0;
+    for (size_t q = 0; q < nq; ++q) {
+      const FINT* s = shls + 4 * q;
+      int64_t blk = ncomp;
+      for (int a = 0; a < naxes; ++a) {
+        const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
+        blk *= (int64_t)(sph ? nsph(bs[ANG_OF]) : ncart(bs[ANG_OF])) * bs[NCTR_OF];
+      }
+      mx = std::max(mx, out_offsets[q] + blk);
+    }
+    p->out_size = mx;
+  }
+  if (out_size) *out_size = p->out_size;
+  p->ntasks = (int64_t)tasks.size();
+  p->stats[4] = nprimq_total;
+  p->stats[5] = p->out_size;
+
+  // ---- counting sort by bucket key ---------------------------------------
+  std::vector<uint32_t> ukeys(key);
+  std::sort(ukeys.begin(), ukeys.end());
+  ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
+  std::vector<int64_t> cnt(ukeys.size() + 1, 0);
+  std::vector<int> kidx(tasks.size());
+  for (size_t t = 0; t < tasks.size(); ++t) {
+    kidx[t] = (int)(std::lower_bound(ukeys.begin(), ukeys.end(), key[t]) - ukeys.begin());
+    cnt[kidx[t] + 1]++;
+  }
+  for (size_t i = 0; i < ukeys.size(); ++i) cnt[i + 1] += cnt[i];
+  std::vector<Task> sorted(tasks.size());
+  std::vector<int> sorted_q(tasks.size());
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (size_t t = 0; t < tasks.size(); ++t) {
+      const int64_t d = pos[kidx[t]]++;
+      sorted[d] = tasks[t];
+      sorted_q[d] = task_q[t];
+    }
+  }
+  int64_t max_gen = 0;
+  for (size_t i = 0; i < ukeys.size(); ++i) {
+    const uint32_t k = ukeys[i];
+    Bucket bk;
+    bk.la = k & 7;
+    bk.lb = (k >> 3) & 7;
+    bk.lc = (k >> 6) & 7;
+    bk.ld = (k >> 9) & 7;
+    bk.perm = (k >> 12) & 255;
+    bk.team = 1 << ((k >> 20) & 15);
+    bk.fast = (k >> 24) & 1;
+    bk.t0 = cnt[i];
+    bk.t1 = cnt[i + 1];
+    bk.sstride = 0;
+    if (!bk.fast) {
+      const GenLayout G = gen_layout(kind, sph, bk.la, bk.lb, bk.lc, bk.ld);
+      if (G.nE > 220 || G.nF > 165) {
+        plan_free(p);
+        set_err("mcint_plan_create: angular momentum beyond the generic kernel's tables");
+        return MCINT_ERR_UNSUPPORTED;
+      }
+      bk.sstride = G.total;
+      const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(bk.t1 - bk.t0,
+                                                   GEN_SCRATCH_DOUBLES / std::max<int64_t>(1, G.total)));
+      max_gen = std::max(max_gen, chunk * G.total);
+      p->stats[3] += bk.t1 - bk.t0;
+    } else {
+      p->stats[2] += bk.t1 - bk.t0;
+    }
+    p->buckets.push_back(bk);
+  }
+  p->stats[0] = p->ntasks;
+  p->stats[1] = (int64_t)p->buckets.size();
+
+  // ---- primitive-pair data of every pair the tasks touch ----------------
+  const int nv = b->nv;
+  std::vector<int64_t> pair_off((size_t)nv * nv, -1);
+  std::vector<int> pairs;
+  int64_t npp = 0;
+  auto touch = [&](int a, int c) {
+    int64_t& o = pair_off[(size_t)a * nv + c];
+    if (o >= 0) return;
+    o = npp;
+    npp += (int64_t)b->vsh[a].nprim * b->vsh[c].nprim;
+    pairs.push_back(a);
+    pairs.push_back(c);
+  };
+  for (const Task& t : sorted) {
+    touch(t.sh[0], t.sh[1]);
+    if (kind != KIND_3C2E) touch(t.sh[2], t.sh[3]);
+  }
+  for (auto& o : pair_off)
+    if (o < 0) o = 0;
+  int* pairs_d = nullptr;
+  hipError_t e = dalloc(&p->tasks_d, sorted.size());
+  if (e == hipSuccess) e = dalloc(&p->task_id_d, sorted_q.size());
+  if (e == hipSuccess) e = dalloc(&p->pair_off_d, pair_off.size());
+  if (e == hipSuccess) e = dalloc(&p->pair_ld_d, pair_off.size());
+  if (e == hipSuccess) e = dalloc(&p->pp_d, (size_t)std::max<int64_t>(npp, 1));
+  if (e == hipSuccess) e = dalloc(&pairs_d, pairs.size());
+  if (e == hipSuccess && max_gen > 0) e = dalloc(&p->scratch_d, (size_t)max_gen);
+  p->scratch_size = max_gen;
+  if (e == hipSuccess && !sorted.empty())
+    e = hipMemcpy(p->tasks_d, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !sorted_q.empty())
+    e = hipMemcpy(p->task_id_d, sorted_q.data(), sorted_q.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->pair_off_d, pair_off.data(), pair_off.size() * sizeof(int64_t),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && !pairs.empty())
+    e = hipMemcpy(pairs_d, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_pairdata(b->vsh_d, b->env_d, nv, pairs_d, (int)(pairs.size() / 2), p->pair_off_d,
+                        p->pair_ld_d, p->pp_d, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  dfree(pairs_d);
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_plan_create: ") + hipGetErrorString(e));
+    plan_free(p);
+    return MCINT_ERR_HIP;
+  }
+  *out = p;
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_stats(const mcint_plan* p, int64_t* n) {
+  if (!p || !n) return MCINT_ERR_ARG;
+  for (int i = 0; i < 6; ++i) n[i] = p->stats[i];
+  return MCINT_OK;
+}
+
+static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream) {
+  mcint_basis* b = p->b;
+  BasisView B;
+  B.vsh = b->vsh_d;
+  B.env = b->env_d;
+  B.pair_off = p->pair_off_d;
+  B.pair_ld = p->pair_ld_d;
+  B.pp = p->pp_d;
+  B.nv = b->nv;
+  B.expcutoff_4c = b->expc4;
+  B.expcutoff_3c = b->expc3;
+  B.omega = b->omega;
+  for (const Bucket& bk : p->buckets) {
+    BucketArgs A;
+    A.tasks = p->tasks_d;
+    A.out = out;
+    A.not0 = not0;
+    A.task_id = p->task_id_d;
+    A.scratch = p->scratch_d;
+    A.scratch_stride = bk.sstride;
+    A.la = bk.la;
+    A.lb = bk.lb;
+    A.lc = bk.lc;
+    A.ld = bk.ld;
+    A.kind = p->kind;
+    A.sph = p->sph;
+    A.perm = bk.perm;
+    A.team = bk.team;
+    if (bk.fast) {
+      A.t0 = bk.t0;
+      A.t1 = bk.t1;
+      HIPCHK(launch_fast(B, A, stream));
+    } else {
+      const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
+      for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+        A.t0 = t;
+        A.t1 = std::min(bk.t1, t + chunk);
+        HIPCHK(launch_generic(B, A, stream));
+      }
+    }
+  }
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_execute_direct(mcint_plan* p, double* out_dev, FINT* not0_dev,
+                                         void* hip_stream) {
+  if (!p || (!out_dev && p->ntasks)) {
+    set_err("mcint_plan_execute: null argument");
+    return MCINT_ERR_ARG;
+  }
+  return plan_launch(p, out_dev, not0_dev, (hipStream_t)hip_stream);
+}
+
+extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev, void* hip_stream) {
+  if (!p || (!out_dev && p->ntasks)) {
+    set_err("mcint_plan_execute: null argument");
+    return MCINT_ERR_ARG;
+  }
+  hipStream_t stream = (hipStream_t)hip_stream;
+  if (!stream) return plan_launch(p, out_dev, not0_dev, stream);
+  if (!p->gexec || p->g_out != out_dev || p->g_not0 != not0_dev || p->g_stream != stream) {
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = plan_launch(p, out_dev, not0_dev, stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st != MCINT_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    HIPCHK(e);
+    p->graph = g;
+    HIPCHK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+    p->g_out = out_dev;
+    p->g_not0 = not0_dev;
+    p->g_stream = stream;
+  }
+  HIPCHK(hipGraphLaunch(p->gexec, stream));
+  return MCINT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer batches
+// ---------------------------------------------------------------------------
+static int host_batch(int intor, double* out, const size_t* out_offsets, const FINT* shls,
+                      size_t nq, const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                      const double* env, FINT* not0, void* hip_stream) {
+  if (!out || (!shls && nq)) {
+    set_err("batch: null argument");
+    return MCINT_ERR_ARG;
+  }
+  if (nq == 0) return MCINT_OK;
+  hipStream_t stream = (hipStream_t)hip_stream;
+  mcint_basis* b = nullptr;
+  int st = mcint_basis_create(&b, atm, natm, bas, nbas, env);
+  if (st) return st;
+  std::vector<int64_t> offs;
+  if (out_offsets) offs.assign(out_offsets, out_offsets + nq);
+  mcint_plan* p = nullptr;
+  int64_t osz = 0;
+  st = mcint_plan_create(&p, b, intor, shls, nq, out_offsets ? offs.data() : nullptr, &osz);
+  if (st) {
+    mcint_basis_destroy(b);
+    return st;
+  }
+  double* od = nullptr;
+  FINT* nd = nullptr;
+  hipError_t e = dalloc(&od, (size_t)std::max<int64_t>(osz, 1));
+  if (e == hipSuccess) e = dalloc(&nd, nq);
+  if (e == hipSuccess && out_offsets) {
+    // blocks not covered by the list keep the caller's values
+    e = hipMemcpyAsync(od, out, (size_t)osz * sizeof(double), hipMemcpyHostToDevice, stream);
+  }
+  if (e == hipSuccess) {
+    st = mcint_plan_execute_direct(p, od, nd, stream);
+    if (st == MCINT_OK) {
+      e = hipMemcpyAsync(out, od, (size_t)osz * sizeof(double), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess && not0)
+        e = hipMemcpyAsync(not0, nd, nq * sizeof(FINT), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
+  }
+  if (e != hipSuccess) {
+    set_err(std::string("batch: ") + hipGetErrorString(e));
+    st = MCINT_ERR_HIP;
+  }
+  dfree(od);
+  dfree(nd);
+  mcint_plan_destroy(p);
+  mcint_basis_destroy(b);
+  return st;
+}
+
+extern "C" int int2e_sph_batch(double* out, const size_t* out_offsets, const FINT* shls, size_t nq,
+                               const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                               const double* env, const CINTOpt* opt, FINT* not0, void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT2E_SPH, out, out_offsets, shls, nq, atm, natm, bas, nbas, env, not0,
+                    stream);
+}
+extern "C" int int2e_cart_batch(double* out, const size_t* out_offsets, const FINT* shls,
+                                size_t nq, const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                                const double* env, const CINTOpt* opt, FINT* not0, void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT2E_CART, out, out_offsets, shls, nq, atm, natm, bas, nbas, env, not0,
+                    stream);
+}
+extern "C" int int2e_ip1_sph_batch(double* out, const size_t* out_offsets, const FINT* shls,
+                                   size_t nq, const FINT* atm, FINT natm, const FINT* bas,
+                                   FINT nbas, const double* env, const CINTOpt* opt, FINT* not0,
+                                   void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT2E_IP1_SPH, out, out_offsets, shls, nq, atm, natm, bas, nbas, env,
+                    not0, stream);
+}
+extern "C" int int3c2e_sph_batch(double* out, const size_t* out_offsets, const FINT* shls,
+                                 size_t nq, const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                                 const double* env, const CINTOpt* opt, FINT* not0, void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT3C2E_SPH, out, out_offsets, shls, nq, atm, natm, bas, nbas, env, not0,
+                    stream);
+}
+
+// ---------------------------------------------------------------------------
+// libcint-named per-call functions
+// ---------------------------------------------------------------------------
+namespace {
+
+struct McOpt {
+  CINTOpt pub;
+  uint64_t magic;
+  int kind;
+};
+constexpr uint64_t OPT_MAGIC = 0x6d63696e744f7074ull;
+
+CACHE_SIZE_T single_call(int intor, double* out, FINT* dims, FINT* shls, FINT* atm, FINT natm,
+                         FINT* bas, FINT nbas, double* env) {
+  const int kind = intor_kind(intor);
+  const int sph = intor_sph(intor);
+  const int naxes = kind == KIND_3C2E ? 3 : 4;
+  const int ncomp = kind == KIND_2E_IP1 ? 3 : 1;
+  if (!shls || !bas) {
+    set_err("null shls/bas");
+    return 0;
+  }
+  int cnt[4] = {1, 1, 1, 1};
+  for (int a = 0; a < naxes; ++a) {
+    if (shls[a] < 0 || shls[a] >= nbas) {
+      set_err("shell index out of range");
+      return 0;
+    }
+    const FINT* bs = bas + (size_t)shls[a] * BAS_SLOTS;
+    cnt[a] = (sph ? nsph(bs[ANG_OF]) : ncart(bs[ANG_OF])) * bs[NCTR_OF];
+  }
+  const int64_t blk = (int64_t)cnt[0] * cnt[1] * cnt[2] * cnt[3];
+  if (!out) {
+    // size hint in doubles, in the spirit of libcint's cache-size query
+    return (CACHE_SIZE_T)std::min<int64_t>(blk * ncomp * 2 + 64, 0x7fffffff);
+  }
+  std::vector<double> tmp((size_t)blk * ncomp);
+  FINT q4[4] = {shls[0], shls[1], shls[2], naxes == 4 ? shls[3] : 0};
+  FINT st = 0;
+  int rc = host_batch(intor, tmp.data(), nullptr, q4, 1, atm, natm, bas, nbas, env, &st, nullptr);
+  if (rc != MCINT_OK) return 0;
+  const FINT* d = dims ? dims : cnt;
+  const int64_t nout = (int64_t)d[0] * d[1] * d[2] * (naxes == 4 ? d[3] : 1);
+  for (int c = 0; c < ncomp; ++c)
+    for (int l = 0; l < cnt[3]; ++l)
+      for (int k = 0; k < cnt[2]; ++k)
+        for (int j = 0; j < cnt[1]; ++j)
+          for (int i = 0; i < cnt[0]; ++i)
+            out[c * nout + i + (int64_t)d[0] * (j + (int64_t)d[1] * (k + (int64_t)d[2] * l))] =
+                tmp[c * blk + i + (int64_t)cnt[0] * (j + (int64_t)cnt[1] * (k + (int64_t)cnt[2] * l))];
+  return st;
+}
+
+void make_opt(CINTOpt** opt, int kind, FINT nbas) {
+  if (!opt) return;
+  auto* o = (McOpt*)calloc(1, sizeof(McOpt));
+  o->pub.nbas = nbas;
+  o->magic = OPT_MAGIC;
+  o->kind = kind;
+  *opt = &o->pub;
+}
+
+}  // namespace
+
+#define MC_INTOR(name, code)                                                                   \
+  extern "C" CACHE_SIZE_T name(double* out, FINT* dims, FINT* shls, FINT* atm, FINT natm,      \
+                               FINT* bas, FINT nbas, double* env, CINTOpt* opt, double* cache) { \
+    (void)opt;                                                                                 \
+    (void)cache;                                                                               \
+    return single_call(code, out, dims, shls, atm, natm, bas, nbas, env);                      \
+  }
+MC_INTOR(int2e_sph, MCINT_INT2E_SPH)
+MC_INTOR(int2e_cart, MCINT_INT2E_CART)
+MC_INTOR(int2e_ip1_sph, MCINT_INT2E_IP1_SPH)
+MC_INTOR(int2e_ip1_cart, MCINT_INT2E_IP1_CART)
+MC_INTOR(int3c2e_sph, MCINT_INT3C2E_SPH)
+MC_INTOR(int3c2e_cart, MCINT_INT3C2E_CART)
+#undef MC_INTOR
+
+extern "C" void int2e_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
+  make_opt(opt, KIND_2E, nbas);
+}
+extern "C" void int2e_ip1_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
+  make_opt(opt, KIND_2E_IP1, nbas);
+}
+extern "C" void int3c2e_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
+  make_opt(opt, KIND_3C2E, nbas);
+}
+extern "C" void CINTdel_optimizer(CINTOpt** opt) {
+  if (!opt || !*opt) return;
+  free((McOpt*)(*opt));
+  *opt = nullptr;
+}
+extern "C" void CINTdel_2e_optimizer(CINTOpt** opt) { CINTdel_optimizer(opt); }
+
+extern "C" FINT cint2e_sph(double* out, FINT* shls, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                           double* env, CINTOpt* opt) {
+  return int2e_sph(out, nullptr, shls, atm, natm, bas, nbas, env, opt, nullptr);
+}
+extern "C" FINT cint2e_cart(double* out, FINT* shls, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                            double* env, CINTOpt* opt) {
+  return int2e_cart(out, nullptr, shls, atm, natm, bas, nbas, env, opt, nullptr);
+}
+extern "C" FINT cint2e_ip1_sph(double* out, FINT* shls, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                               double* env, CINTOpt* opt) {
+  return int2e_ip1_sph(out, nullptr, shls, atm, natm, bas, nbas, env, opt, nullptr);
+}
+extern "C" void cint2e_sph_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                     double* env) {
+  int2e_optimizer(opt, atm, natm, bas, nbas, env);
+}
+
+// ---------------------------------------------------------------------------
+// Basis helpers (src/misc.c:72-93, src/cint_bas.c:36-160)
+// ---------------------------------------------------------------------------
+extern "C" double CINTgto_norm(FINT n, double a) {
+  const double n1 = n + 1.5;
+  return 1.0 / std::sqrt(std::exp(std::lgamma(n1)) / (2.0 * std::pow(2.0 * a, n1)));
+}
+extern "C" FINT CINTcgto_spheric(const FINT bas_id, const FINT* bas) {
+  return (bas[bas_id * BAS_SLOTS + ANG_OF] * 2 + 1) * bas[bas_id * BAS_SLOTS + NCTR_OF];
+}
+extern "C" FINT CINTcgto_cart(const FINT bas_id, const FINT* bas) {
+  const int l = bas[bas_id * BAS_SLOTS + ANG_OF];
+  return (l + 1) * (l + 2) / 2 * bas[bas_id * BAS_SLOTS + NCTR_OF];
+}
+extern "C" FINT CINTtot_cgto_spheric(const FINT* bas, const FINT nbas) {
+  FINT s = 0;
+  for (int i = 0; i < nbas; ++i) s += CINTcgto_spheric(i, bas);
+  return s;
+}
+extern "C" FINT CINTtot_cgto_cart(const FINT* bas, const FINT nbas) {
+  FINT s = 0;
+  for (int i = 0; i < nbas; ++i) s += CINTcgto_cart(i, bas);
+  return s;
+}
+extern "C" void CINTshells_spheric_offset(FINT ao_loc[], const FINT* bas, const FINT nbas) {
+  ao_loc[0] = 0;
+  for (int i = 1; i < nbas; ++i) ao_loc[i] = ao_loc[i - 1] + CINTcgto_spheric(i - 1, bas);
+}
+extern "C" void CINTshells_cart_offset(FINT ao_loc[], const FINT* bas, const FINT nbas) {
+  ao_loc[0] = 0;
+  for (int i = 1; i < nbas; ++i) ao_loc[i] = ao_loc[i - 1] + CINTcgto_cart(i - 1, bas);
+}
+
+// ---------------------------------------------------------------------------
+// Rys roots on the device
+// ---------------------------------------------------------------------------
+extern "C" int mcint_rys_roots_batch(int nroots, const double* x, size_t n, double* u, double* w) {
+  if (nroots < 1 || nroots > NRYS_MAX || (!x && n)) {
+    set_err("mcint_rys_roots_batch: nroots must be 1..9");
+    return MCINT_ERR_ARG;
+  }
+  if (n == 0) return MCINT_OK;
+  for (size_t i = 0; i < n; ++i)
+    if (!(x[i] >= 0) || std::isinf(x[i])) {
+      set_err("mcint_rys_roots_batch: x must be finite and >= 0");
+      return MCINT_ERR_ARG;
+    }
+  double *xd = nullptr, *ud = nullptr, *wd = nullptr;
+  hipError_t e = dalloc(&xd, n);
+  if (e == hipSuccess) e = dalloc(&ud, n * nroots);
+  if (e == hipSuccess) e = dalloc(&wd, n * nroots);
+  if (e == hipSuccess) e = hipMemcpy(xd, x, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_rys_batch(nroots, xd, (int64_t)n, ud, wd, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(u, ud, n * nroots * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(w, wd, n * nroots * sizeof(double), hipMemcpyDeviceToHost);
+  dfree(xd);
+  dfree(ud);
+  dfree(wd);
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_rys_roots_batch: ") + hipGetErrorString(e));
+    return MCINT_ERR_HIP;
+  }
+  return MCINT_OK;
+}
+
+extern "C" int CINTrys_roots(int nroots, double x, double* u, double* w) {
+  return mcint_rys_roots_batch(nroots, &x, 1, u, w);
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz bounds
+// ---------------------------------------------------------------------------
+extern "C" int mcint_schwarz(mcint_basis* b, int nshell, double* q) {
+  if (!b || !q) {
+    set_err("mcint_schwarz: null argument");
+    return MCINT_ERR_ARG;
+  }
+  const int nb = b->nbas;
+  if (nshell <= 0 || nshell > nb) nshell = nb;
+  std::vector<FINT> shls;
+  for (int i = 0; i < nshell; ++i)
+    for (int j = 0; j <= i; ++j) {
+      shls.push_back(i);
+      shls.push_back(j);
+      shls.push_back(i);
+      shls.push_back(j);
+    }
+  const size_t nq = shls.size() / 4;
+  mcint_plan* p = nullptr;
+  int64_t osz = 0;
+  int st = mcint_plan_create(&p, b, MCINT_INT2E_SPH, shls.data(), nq, nullptr, &osz);
+  if (st) return st;
+  double* od = nullptr;
+  hipError_t e = dalloc(&od, (size_t)osz);
+  std::vector<double> h((size_t)osz);
+  if (e == hipSuccess) {
+    st = mcint_plan_execute_direct(p, od, nullptr, nullptr);
+    if (st == MCINT_OK) e = hipMemcpy(h.data(), od, (size_t)osz * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  dfree(od);
+  mcint_plan_destroy(p);
+  if (st) return st;
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_schwarz: ") + hipGetErrorString(e));
+    return MCINT_ERR_HIP;
+  }
+  for (size_t i = 0; i < (size_t)nb * nb; ++i) q[i] = 0;
+  size_t off = 0, k = 0;
+  for (int i = 0; i < nshell; ++i)
+    for (int j = 0; j <= i; ++j, ++k) {
+      const int di = CINTcgto_spheric(i, b->bas.data()), dj = CINTcgto_spheric(j, b->bas.data());
+      const size_t n = (size_t)di * dj * di * dj;
+      double m = 0;
+      for (size_t t = 0; t < n; ++t) m = std::max(m, std::fabs(h[off + t]));
+      off += n;
+      q[(size_t)i * nb + j] = q[(size_t)j * nb + i] = std::sqrt(m);
+    }
+  return MCINT_OK;
+}
+
+extern "C" const char* mcint_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,529 @@
+// Device code of libmcint: pair data, the generic Rys kernel (any class,
+// plain / i-derivative / 3-centre), and the register-resident fast kernels.
+//
+// Algorithm (the mathematics of libcint's Rys path, src/g2e.c:4425-4545,
+// reorganised for the GPU):
+//   per primitive quartet: Rys roots (tables), the Rys coefficients
+//   b00/b10/b01/c00/c0p, the 2D vertical recurrence I(e,f) with e built on
+//   centre A and f on centre C (src/g2e.c:272-421), and the products
+//   [e0|f0] = sum_roots Ix*Iy*Iz accumulated over primitives;
+//   per contracted quartet: the horizontal transfers (a,b| <- [e0| and
+//   |c,d) <- |f0] done once on contracted data (libcint does them per
+//   primitive, src/g2e.c:428-693), the Cartesian -> spherical transform
+//   (src/cart2sph.c:5324-5376) and the store in libcint's Fortran order.
+#include "eri_launch.h"
+#include "rys_device.h"
+
+#define C2S_QUAL static __device__ const
+#include "c2s_tables.inc"
+#undef C2S_QUAL
+
+namespace mcint {
+
+__device__ __forceinline__ double common_fac_sp(int l) {
+  // 1/(2 sqrt(pi)) and sqrt(3/(4 pi)); src/g1e.c:565-572
+  return l == 0 ? 0.282094791773878143 : (l == 1 ? 0.488602511902919921 : 1.0);
+}
+
+constexpr double PI3_2_SQRTPI = 34.986836655249725;  // 2 pi^3 / sqrt(pi)
+
+__device__ __forceinline__ int cart_idx(int l, int lx, int lz) {
+  const int m = l - lx;
+  return m * (m + 1) / 2 + lz;
+}
+
+// ---------------------------------------------------------------------
+// Pair data: one block per ordered virtual-shell pair, one lane per
+// primitive pair.  Restates CINTset_pairdata (src/optimizer.c:288-342).
+// ---------------------------------------------------------------------
+__global__ void pairdata_kernel(const VShell* __restrict__ vsh, const double* __restrict__ env,
+                                int nv, const int* __restrict__ pairs,
+                                const int64_t* __restrict__ pair_off, double* __restrict__ pair_ld,
+                                PairPrim* __restrict__ pp) {
+  const int ia = pairs[2 * blockIdx.x];
+  const int ib = pairs[2 * blockIdx.x + 1];
+  const VShell a = vsh[ia];
+  const VShell b = vsh[ib];
+  const double dx = a.r[0] - b.r[0], dy = a.r[1] - b.r[1], dz = a.r[2] - b.r[2];
+  const double rr = dx * dx + dy * dy + dz * dz;
+  const double* ea = env + a.pexp;
+  const double* eb = env + b.pexp;
+  const double log_rr0 = 1.7 - 1.5 * log(ea[a.nprim - 1] + eb[b.nprim - 1]);
+  if (threadIdx.x == 0) pair_ld[(int64_t)ia * nv + ib] = log(sqrt(rr) + 1.0);
+  const int64_t base = pair_off[(int64_t)ia * nv + ib];
+  const int n = a.nprim * b.nprim;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int ip = p % a.nprim;
+    const int jp = p / a.nprim;
+    double mca = 0, mcb = 0;
+    for (int c = 0; c < a.nctr; ++c) mca = fmax(mca, fabs(env[a.pcoef0 + c * a.nprim + ip]));
+    for (int c = 0; c < b.nctr; ++c) mcb = fmax(mcb, fabs(env[b.pcoef0 + c * b.nprim + jp]));
+    const double ai = ea[ip], aj = eb[jp];
+    const double aij = ai + aj;
+    const double inv = 1.0 / aij;
+    const double e = rr * ai * aj * inv;
+    PairPrim q;
+    const double wj = aj * inv;
+    q.px = a.r[0] + wj * (b.r[0] - a.r[0]);
+    q.py = a.r[1] + wj * (b.r[1] - a.r[1]);
+    q.pz = a.r[2] + wj * (b.r[2] - a.r[2]);
+    q.aij = aij;
+    q.kab = env[a.pcoef + ip] * env[b.pcoef + jp] * exp(-e);
+    q.cce0 = e - log_rr0 - log(mca) - log(mcb);
+    q.ai = ai;
+    q.aj = aj;
+    pp[base + p] = q;
+  }
+}
+
+hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const int* pairs,
+                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* pp,
+                           hipStream_t stream) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pairdata_kernel, dim3(npairs), dim3(64), 0, stream, vsh, env, nv, pairs,
+                     pair_off, pair_ld, pp);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// Output addressing shared by all kernels.  Canonical axis A holds the
+// original axis (perm >> 2A) & 3; strides are those of the packed
+// Fortran block (D0, D0*D1, ...) of the original shell order.
+// ---------------------------------------------------------------------
+struct OutStrides {
+  int64_t s[4];
+  int64_t comp;
+};
+
+__device__ __forceinline__ OutStrides out_strides(const VShell* vsh, const Task& tk, int perm,
+                                                  int naxes, int sph) {
+  int dorig[4] = {1, 1, 1, 1};
+  for (int A = 0; A < naxes; ++A) {
+    const VShell& v = vsh[tk.sh[A]];
+    dorig[(perm >> (2 * A)) & 3] = sph ? v.dsph : v.dcart;
+  }
+  int64_t so[5];
+  so[0] = 1;
+  for (int o = 0; o < 4; ++o) so[o + 1] = so[o] * dorig[o];
+  OutStrides st;
+  for (int A = 0; A < 4; ++A) st.s[A] = A < naxes ? so[(perm >> (2 * A)) & 3] : 0;
+  st.comp = so[4];
+  return st;
+}
+
+// ---------------------------------------------------------------------
+// Generic kernel: one 64-lane workgroup per contracted quartet.  Roots,
+// recurrences and accumulation are spread over lanes through LDS and a
+// per-task scratch block; used for every class without a fast kernel.
+// ---------------------------------------------------------------------
+constexpr int GEN_LANES = 64;
+constexpr int GEN_VCAP = 3 * NRYS_MAX * 90;  // e+f <= 17 bounds (E+1)(F+1) by 90
+constexpr int GEN_ECAP = 220;                // sum of ncart(l), l <= 9
+constexpr int GEN_FCAP = 165;                // l <= 8
+
+__device__ __forceinline__ double binom_small(int n, int k) {
+  double r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+__device__ __forceinline__ double ipow(double x, int k) {
+  double r = 1;
+  for (int i = 0; i < k; ++i) r *= x;
+  return r;
+}
+
+// Horizontal transfer (a,b| = sum_t prod_d C(b_d,t_d) AB_d^(b_d-t_d) [a+t|
+// for one a (cart exps ax..az) and b, reading column f of a set whose
+// e-list starts at shell set_lo.
+__device__ double hrr_bra(const double* __restrict__ X, int nF, int f, int set_lo, int ax, int ay,
+                          int az, int bx, int by, int bz, const double* AB) {
+  double s = 0;
+  for (int tx = 0; tx <= bx; ++tx) {
+    const double cx = binom_small(bx, tx) * ipow(AB[0], bx - tx);
+    for (int ty = 0; ty <= by; ++ty) {
+      const double cy = cx * binom_small(by, ty) * ipow(AB[1], by - ty);
+      for (int tz = 0; tz <= bz; ++tz) {
+        const double cz = cy * binom_small(bz, tz) * ipow(AB[2], bz - tz);
+        const int ex = ax + tx, ey = ay + ty, ez = az + tz;
+        const int le = ex + ey + ez;
+        const int e = cart_sum(set_lo, le - 1) + cart_idx(le, ex, ez);
+        s += cz * X[(int64_t)e * nF + f];
+      }
+    }
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(GEN_LANES)
+eri_generic_kernel(BasisView B, BucketArgs A) {
+  __shared__ double s_u[NRYS_MAX], s_w[NRYS_MAX];
+  __shared__ double s_V[GEN_VCAP];
+  __shared__ int s_ec[GEN_ECAP];
+  __shared__ int s_fc[GEN_FCAP];
+
+  const int64_t t = A.t0 + blockIdx.x;
+  if (t >= A.t1) return;
+  const int lane = threadIdx.x;
+  const Task tk = A.tasks[t];
+  const int kind = A.kind;
+  const bool ip1 = kind == KIND_2E_IP1;
+  const bool three = kind == KIND_3C2E;
+  const int la = A.la, lb = A.lb, lc = A.lc, ld = three ? 0 : A.ld;
+  const GenLayout G = gen_layout(kind, A.sph, la, lb, lc, ld);
+  double* X = A.scratch + (t - A.t0) * A.scratch_stride;
+
+  const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
+  VShell s3 = s2;
+  if (!three) s3 = B.vsh[tk.sh[3]];
+  const OutStrides OS = out_strides(B.vsh, tk, A.perm, three ? 3 : 4, A.sph);
+  double* out = A.out + tk.out;
+
+  // component tables of the union e-list and the f-list
+  for (int i = lane; i < G.nE; i += GEN_LANES) {
+    int l = G.e_lo, k = i;
+    while (k >= ncart(l)) { k -= ncart(l); ++l; }
+    // k-th component of shell l in libcint order
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= k) ++m;
+    const int lz = k - m * (m + 1) / 2;
+    const int lx = l - m;
+    const int ly = m - lz;
+    s_ec[i] = lx | (ly << 4) | (lz << 8) | (l << 12);
+  }
+  for (int i = lane; i < G.nF; i += GEN_LANES) {
+    int l = lc, k = i;
+    while (k >= ncart(l)) { k -= ncart(l); ++l; }
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= k) ++m;
+    const int lz = k - m * (m + 1) / 2;
+    s_fc[i] = (l - m) | ((m - lz) << 4) | (lz << 8) | (l << 12);
+  }
+  for (int64_t i = lane; i < G.oH1; i += GEN_LANES) X[i] = 0;
+
+  const int nij = s0.nprim * s1.nprim;
+  const int nkl = three ? s2.nprim : s2.nprim * s3.nprim;
+  const PairPrim* pij = B.pp + B.pair_off[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
+  const PairPrim* pkl = three ? nullptr : B.pp + B.pair_off[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
+  const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
+  const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
+  const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
+  const int lij = la + lb + (ip1 ? 1 : 0);
+  const int lkl = lc + ld;
+
+  // pair emptiness (libcint returns early when a pair has no primitive
+  // below the cutoff: src/cint2e.c:96-109)
+  int live_ij = 0, live_kl = three ? 1 : 0;
+  for (int p = lane; p < nij; p += GEN_LANES) live_ij |= (pij[p].cce0 - lij * ld_ij) < E;
+  if (!three)
+    for (int p = lane; p < nkl; p += GEN_LANES) live_kl |= (pkl[p].cce0 - lkl * ld_kl) < E;
+  const int live = __syncthreads_and(__syncthreads_or(live_ij) && __syncthreads_or(live_kl));
+
+  const int nr = (la + lb + lc + ld + (ip1 ? 1 : 0)) / 2 + 1;
+  const int VS = (G.E + 1) * (G.F + 1);
+  const double cfac = PI3_2_SQRTPI * common_fac_sp(A.la) * common_fac_sp(A.lb) *
+                      common_fac_sp(A.lc) * (three ? 1.0 : common_fac_sp(ld));
+  const double Ax = s0.r[0], Ay = s0.r[1], Az = s0.r[2];
+  const double Cx = s2.r[0], Cy = s2.r[1], Cz = s2.r[2];
+  const double omega = B.omega;
+  const double* ek = B.env + s2.pexp;
+  const double* ck = B.env + s2.pcoef;
+  int any = 0;
+
+  if (live) {
+    const int ncomp_g = G.nE * G.nF;
+    for (int p = 0; p < nij * nkl; ++p) {
+      const int ij = p % nij;
+      const int kl = p / nij;
+      const PairPrim P = pij[ij];
+      double cce_kl, akl, qx, qy, qz, kkl;
+      if (three) {
+        cce_kl = 0;
+        akl = ek[kl];
+        qx = Cx; qy = Cy; qz = Cz;
+        kkl = ck[kl];
+      } else {
+        const PairPrim Q = pkl[kl];
+        cce_kl = Q.cce0 - lkl * ld_kl;
+        akl = Q.aij;
+        qx = Q.px; qy = Q.py; qz = Q.pz;
+        kkl = Q.kab;
+        if (cce_kl > E) continue;
+      }
+      const double cce_ij = P.cce0 - lij * ld_ij;
+      if (cce_ij > (three ? E : E - cce_kl)) continue;
+      any = 1;
+      if (cce_ij >= E || cce_kl >= E) continue;  // stored exp factor is 0
+
+      const double aij = P.aij;
+      const double a1 = aij * akl;
+      const double a0 = a1 / (aij + akl);
+      const double xpq = P.px - qx, ypq = P.py - qy, zpq = P.pz - qz;
+      const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
+      double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * kkl;
+      double x = a0 * rr;
+      double theta = 1.0;
+      if (omega > 0) {
+        theta = omega * omega / (omega * omega + a0);
+        x *= theta;
+        fac1 *= sqrt(theta);
+      }
+      if (lane < nr) {
+        double u, w;
+        rys_root_one(nr, lane, x, u, w);
+        if (omega > 0) {
+          const double ut = u * theta;
+          u = ut / (u + 1.0 - ut);
+        }
+        s_u[lane] = u;
+        s_w[lane] = w * fac1;
+      }
+      __syncthreads();
+      if (lane < 3 * nr) {
+        const int d = lane / nr;
+        const int r = lane - d * nr;
+        const double u2 = a0 * s_u[r];
+        const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+        const double tmp5 = u2 * tmp4;
+        const double tmp1 = 2.0 * tmp5;
+        const double b00 = tmp5;
+        const double b10 = tmp5 + tmp4 * akl;
+        const double b01 = tmp5 + tmp4 * aij;
+        const double pq = d == 0 ? xpq : (d == 1 ? ypq : zpq);
+        const double pa = d == 0 ? P.px - Ax : (d == 1 ? P.py - Ay : P.pz - Az);
+        const double qc = d == 0 ? qx - Cx : (d == 1 ? qy - Cy : qz - Cz);
+        const double c00 = pa - tmp1 * akl * pq;
+        const double c0p = qc + tmp1 * aij * pq;
+        double* V = s_V + (d * nr + r) * VS;
+        const int F1 = G.F + 1;
+        V[0] = d == 2 ? s_w[r] : 1.0;
+        for (int e = 0; e < G.E; ++e) {
+          double v = c00 * V[e * F1];
+          if (e > 0) v += e * b10 * V[(e - 1) * F1];
+          V[(e + 1) * F1] = v;
+        }
+        for (int f = 0; f < G.F; ++f) {
+          for (int e = 0; e <= G.E; ++e) {
+            double v = c0p * V[e * F1 + f];
+            if (f > 0) v += f * b01 * V[e * F1 + f - 1];
+            if (e > 0) v += e * b00 * V[(e - 1) * F1 + f];
+            V[e * F1 + f + 1] = v;
+          }
+        }
+      }
+      __syncthreads();
+      const double wai = 2.0 * P.ai;
+      for (int c = lane; c < ncomp_g; c += GEN_LANES) {
+        const int ie = c / G.nF;
+        const int jf = c - ie * G.nF;
+        const int ec = s_ec[ie], fc = s_fc[jf];
+        const int ex = ec & 15, ey = (ec >> 4) & 15, ez = (ec >> 8) & 15, le = ec >> 12;
+        const int fx = fc & 15, fy = (fc >> 4) & 15, fz = (fc >> 8) & 15;
+        const int F1 = G.F + 1;
+        double g = 0;
+        for (int r = 0; r < nr; ++r) {
+          g += s_V[(0 * nr + r) * VS + ex * F1 + fx] * s_V[(1 * nr + r) * VS + ey * F1 + fy] *
+               s_V[(2 * nr + r) * VS + ez * F1 + fz];
+        }
+        if (le >= G.p_lo && le <= G.p_hi) {
+          const int e0 = cart_sum(G.p_lo, le - 1) + cart_idx(le, ex, ez);
+          X[G.oX0 + (int64_t)e0 * G.nF + jf] += ip1 ? wai * g : g;
+        }
+        if (le >= G.m_lo && le <= G.m_hi) {
+          const int e1 = cart_sum(G.m_lo, le - 1) + cart_idx(le, ex, ez);
+          X[G.oXM + (int64_t)e1 * G.nF + jf] += g;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  any = __syncthreads_or(any) && live;
+  if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
+
+  if (!any) {
+    const int n = G.ncomp * G.nAo * G.nBo * G.nCo * (three ? 1 : G.nDo);
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int a = k % G.nAo; k /= G.nAo;
+      const int b = k % G.nBo; k /= G.nBo;
+      const int c = k % G.nCo; k /= G.nCo;
+      const int d = three ? 0 : k % G.nDo;
+      const int comp = three ? k : k / G.nDo;
+      out[comp * OS.comp + a * OS.s[0] + b * OS.s[1] + c * OS.s[2] + (three ? 0 : d * OS.s[3])] = 0;
+    }
+    return;
+  }
+
+  // pass 1: bra transfer (Cartesian), H1[comp][a][b][f]
+  const double AB[3] = {s0.r[0] - s1.r[0], s0.r[1] - s1.r[1], s0.r[2] - s1.r[2]};
+  {
+    const int n = G.ncomp * G.nca * G.ncb * G.nF;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int f = k % G.nF; k /= G.nF;
+      const int b = k % G.ncb; k /= G.ncb;
+      const int a = k % G.nca;
+      const int comp = k / G.nca;
+      // cart exponents of a (shell la) and b (shell lb)
+      int m = 0;
+      while ((m + 1) * (m + 2) / 2 <= a) ++m;
+      const int az = a - m * (m + 1) / 2, ax = la - m, ay = m - az;
+      int mb = 0;
+      while ((mb + 1) * (mb + 2) / 2 <= b) ++mb;
+      const int bz = b - mb * (mb + 1) / 2, bx = lb - mb, by = mb - bz;
+      double v;
+      if (!ip1) {
+        v = hrr_bra(X + G.oX0, G.nF, f, G.p_lo, ax, ay, az, bx, by, bz, AB);
+      } else {
+        // (d/dr_comp a b| = a_comp (a-1|b) - [2 a_i weighted] (a+1|b)
+        const int dx = comp == 0, dy = comp == 1, dz = comp == 2;
+        v = -hrr_bra(X + G.oX0, G.nF, f, G.p_lo, ax + dx, ay + dy, az + dz, bx, by, bz, AB);
+        const int ac = comp == 0 ? ax : (comp == 1 ? ay : az);
+        if (ac > 0)
+          v += ac * hrr_bra(X + G.oXM, G.nF, f, G.m_lo, ax - dx, ay - dy, az - dz, bx, by, bz, AB);
+      }
+      X[G.oH1 + i] = v;
+    }
+  }
+  __syncthreads();
+  // pass 2: bra Cartesian -> spherical, H2[comp][ao][bo][f]
+  {
+    const int n = G.ncomp * G.nAo * G.nBo * G.nF;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int f = k % G.nF; k /= G.nF;
+      const int bo = k % G.nBo; k /= G.nBo;
+      const int ao = k % G.nAo;
+      const int comp = k / G.nAo;
+      const double* h1 = X + G.oH1 + (int64_t)comp * G.nca * G.ncb * G.nF;
+      double v;
+      if (!A.sph) {
+        v = h1[((int64_t)ao * G.ncb + bo) * G.nF + f];
+      } else {
+        const double* ca = c2s_coef + c2s_offset[la] + ao * G.nca;
+        const double* cb = c2s_coef + c2s_offset[lb] + bo * G.ncb;
+        v = 0;
+        for (int a = 0; a < G.nca; ++a) {
+          if (ca[a] == 0.0) continue;
+          double s = 0;
+          for (int b = 0; b < G.ncb; ++b)
+            if (cb[b] != 0.0) s += cb[b] * h1[((int64_t)a * G.ncb + b) * G.nF + f];
+          v += ca[a] * s;
+        }
+      }
+      X[G.oH2 + i] = v;
+    }
+  }
+  __syncthreads();
+  const double* H3;
+  int n_c;  // Cartesian width of the ket before its spherical transform
+  if (!three) {
+    // pass 3: ket transfer (Cartesian), H3[comp][ao][bo][c][d]
+    const double CD[3] = {s2.r[0] - s3.r[0], s2.r[1] - s3.r[1], s2.r[2] - s3.r[2]};
+    const int n = G.ncomp * G.nAo * G.nBo * G.ncc * G.ncd;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int d = k % G.ncd; k /= G.ncd;
+      const int c = k % G.ncc; k /= G.ncc;
+      const int row = k;  // comp*nAo*nBo + ao*nBo + bo
+      int m = 0;
+      while ((m + 1) * (m + 2) / 2 <= c) ++m;
+      const int cz = c - m * (m + 1) / 2, cx = lc - m, cy = m - cz;
+      int md = 0;
+      while ((md + 1) * (md + 2) / 2 <= d) ++md;
+      const int dz = d - md * (md + 1) / 2, dx = ld - md, dy = md - dz;
+      // ket transfer reads f-columns of one H2 row
+      const double* h2 = X + G.oH2 + (int64_t)row * G.nF;
+      double s = 0;
+      for (int tx = 0; tx <= dx; ++tx) {
+        const double wx = binom_small(dx, tx) * ipow(CD[0], dx - tx);
+        for (int ty = 0; ty <= dy; ++ty) {
+          const double wy = wx * binom_small(dy, ty) * ipow(CD[1], dy - ty);
+          for (int tz = 0; tz <= dz; ++tz) {
+            const double wz = wy * binom_small(dz, tz) * ipow(CD[2], dz - tz);
+            const int fx = cx + tx, fy = cy + ty, fz = cz + tz;
+            const int lf = fx + fy + fz;
+            const int f = cart_sum(lc, lf - 1) + cart_idx(lf, fx, fz);
+            s += wz * h2[f];
+          }
+        }
+      }
+      X[G.oH3 + i] = s;
+    }
+    __syncthreads();
+    H3 = X + G.oH3;
+    n_c = G.ncc * G.ncd;
+  } else {
+    H3 = X + G.oH2;
+    n_c = G.nF;  // = ncart(lc)
+  }
+  // pass 4: ket Cartesian -> spherical and the store
+  {
+    const int nD = three ? 1 : G.nDo;
+    const int n = G.ncomp * G.nAo * G.nBo * G.nCo * nD;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int ao = k % G.nAo; k /= G.nAo;
+      const int bo = k % G.nBo; k /= G.nBo;
+      const int co = k % G.nCo; k /= G.nCo;
+      const int dd = k % nD;
+      const int comp = k / nD;
+      const int row = (comp * G.nAo + ao) * G.nBo + bo;
+      const double* h3 = H3 + (int64_t)row * n_c;
+      double v;
+      if (!A.sph) {
+        v = three ? h3[co] : h3[co * G.ncd + dd];
+      } else if (three) {
+        const double* cc = c2s_coef + c2s_offset[lc] + co * G.ncc;
+        v = 0;
+        for (int c = 0; c < G.ncc; ++c)
+          if (cc[c] != 0.0) v += cc[c] * h3[c];
+      } else {
+        const double* cc = c2s_coef + c2s_offset[lc] + co * G.ncc;
+        const double* cd = c2s_coef + c2s_offset[ld] + dd * G.ncd;
+        v = 0;
+        for (int c = 0; c < G.ncc; ++c) {
+          if (cc[c] == 0.0) continue;
+          double s = 0;
+          for (int d = 0; d < G.ncd; ++d)
+            if (cd[d] != 0.0) s += cd[d] * h3[c * G.ncd + d];
+          v += cc[c] * s;
+        }
+      }
+      out[comp * OS.comp + ao * OS.s[0] + bo * OS.s[1] + co * OS.s[2] + (three ? 0 : dd * OS.s[3])] = v;
+    }
+  }
+}
+
+__global__ void rys_batch_kernel(int n, const double* __restrict__ x, int64_t count,
+                                 double* __restrict__ u, double* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count * n) return;
+  const int64_t k = i / n;
+  const int r = (int)(i - k * n);
+  double uu, ww;
+  rys_root_one(n, r, x[k], uu, ww);
+  u[k * n + r] = uu;
+  w[k * n + r] = ww;
+}
+
+hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, double* w,
+                            hipStream_t stream) {
+  const int64_t tot = count * n;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rys_batch_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                     n, x, count, u, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream) {
+  const int64_t n = A.t1 - A.t0;
+  if (n <= 0) return hipSuccess;
+  // grid.x is limited to 2^31-1 workgroups; callers split larger buckets
+  hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)n), dim3(GEN_LANES), 0, stream, B, A);
+  return hipGetLastError();
+}
+
+}  // namespace mcint
+
+#include "eri_fast.h"

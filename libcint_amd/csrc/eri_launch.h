@@ -1,0 +1,87 @@
+// Host-side entry points of the device code (eri.hip).
+#pragma once
+#include "mcint_internal.h"
+
+namespace mcint {
+
+// Scratch layout of the generic (any class) kernel, per task, in doubles.
+// X0: primitive-contracted [e0|f0] (plain), or the 2*a_i weighted set of
+//     the i-derivative; XM: the unweighted lowered set of the derivative;
+// H1: bra transfer (Cartesian); H2: bra Cartesian->spherical;
+// H3: ket transfer (Cartesian, 4-centre only).
+struct GenLayout {
+  int e_lo, E, F;         // e shells e_lo..E, f shells lc..F
+  int nE, nF;             // union e-list and f-list lengths
+  int p_lo, p_hi;         // shells of the main (or 2a-weighted) set
+  int m_lo, m_hi;         // shells of the lowered set (m_hi < m_lo: empty)
+  int NE0, NEM;           // lengths of the two e-sets
+  int ncomp;
+  int nca, ncb, nAo, nBo, ncc, ncd, nCo, nDo;
+  int64_t oX0, oXM, oH1, oH2, oH3, total;
+};
+
+__host__ __device__ inline int cart_sum(int lo, int hi) {
+  int s = 0;
+  for (int l = lo; l <= hi; ++l) s += ncart(l);
+  return s;
+}
+
+__host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int lb, int lc, int ld) {
+  GenLayout g;
+  const bool ip1 = kind == KIND_2E_IP1;
+  const bool three = kind == KIND_3C2E;
+  if (three) ld = 0;
+  g.ncomp = ip1 ? 3 : 1;
+  if (ip1) {
+    g.p_lo = la + 1;
+    g.p_hi = la + lb + 1;
+    g.m_lo = la - 1;
+    g.m_hi = la + lb - 1;
+    if (la == 0) { g.m_lo = 1; g.m_hi = 0; }
+    g.e_lo = la > 0 ? la - 1 : la + 1;
+    g.E = la + lb + 1;
+  } else {
+    g.p_lo = la;
+    g.p_hi = la + lb;
+    g.m_lo = 1;
+    g.m_hi = 0;
+    g.e_lo = la;
+    g.E = la + lb;
+  }
+  g.F = lc + ld;
+  g.nE = cart_sum(g.e_lo, g.E);
+  g.nF = cart_sum(lc, g.F);
+  g.NE0 = cart_sum(g.p_lo, g.p_hi);
+  g.NEM = g.m_hi >= g.m_lo ? cart_sum(g.m_lo, g.m_hi) : 0;
+  g.nca = ncart(la);
+  g.ncb = ncart(lb);
+  g.ncc = ncart(lc);
+  g.ncd = ncart(ld);
+  g.nAo = sph ? nsph(la) : g.nca;
+  g.nBo = sph ? nsph(lb) : g.ncb;
+  g.nCo = sph ? nsph(lc) : g.ncc;
+  g.nDo = sph ? nsph(ld) : g.ncd;
+  g.oX0 = 0;
+  g.oXM = g.oX0 + (int64_t)g.NE0 * g.nF;
+  g.oH1 = g.oXM + (int64_t)g.NEM * g.nF;
+  g.oH2 = g.oH1 + (int64_t)g.ncomp * g.nca * g.ncb * g.nF;
+  g.oH3 = g.oH2 + (int64_t)g.ncomp * g.nAo * g.nBo * g.nF;
+  g.total = g.oH3 + (three ? 0 : (int64_t)g.ncomp * g.nAo * g.nBo * g.ncc * g.ncd);
+  return g;
+}
+
+hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const int* pairs,
+                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* pp,
+                           hipStream_t stream);
+
+hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, double* w,
+                            hipStream_t stream);
+
+hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+
+// Register-resident kernels for the classes listed in eri_fast.h.
+bool fast_kernel_available(int kind, int sph, int la, int lb, int lc, int ld);
+int fast_team_lanes_max();
+hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+
+}  // namespace mcint

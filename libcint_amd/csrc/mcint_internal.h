@@ -1,0 +1,89 @@
+// Internal data structures shared by the HIP kernels and the host driver of
+// libmcint (MI355X / gfx950).  Nothing here crosses the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcint {
+
+// env / atm / bas slots of the libcint data model (include/cint.h.in:26-67)
+constexpr int PTR_EXPCUTOFF = 0;
+constexpr int PTR_RANGE_OMEGA = 8;
+constexpr int ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2, NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6;
+constexpr int BAS_SLOTS = 8, ATM_SLOTS = 6, PTR_COORD = 1;
+
+constexpr double EXPCUTOFF = 60.0;      // src/cint_config.h.in:27
+constexpr double MIN_EXPCUTOFF = 40.0;  // src/cint_config.h.in:30
+constexpr int LMAX = 6;                 // largest shell l handled on the device
+constexpr int NRYS_MAX = 9;             // largest Rys root count of the tables
+
+enum IntorKind : int { KIND_2E = 0, KIND_2E_IP1 = 1, KIND_3C2E = 2 };
+
+// One "virtual shell": a (shell, contraction column) pair.  Generally
+// contracted shells expand into nctr virtual shells, each a plain
+// segmented contraction; the output sub-block of column c starts at
+// c * nsph inside the real shell's (nsph * nctr) range.
+struct VShell {
+  double r[3];     // centre (Bohr)
+  int l;
+  int nprim;
+  int pexp;        // offset of the exponents in env
+  int pcoef;       // offset of this column's coefficients in env
+  int pcoef0;      // offset of the real shell's first coefficient column
+  int nctr;        // contraction columns of the real shell
+  int dsph;        // output width of the real shell, spherical: (2l+1) * nctr
+  int dcart;       // output width of the real shell, Cartesian
+  int shell;       // index of the real shell
+};
+
+// Primitive-pair data of an ordered virtual-shell pair, primitive i fastest.
+// cce0 is libcint's screening estimate without the l-dependent distance
+// term (src/optimizer.c:288-342):  cce = cce0 - (li+lj) * pair_ld.
+struct PairPrim {
+  double px, py, pz;  // Gaussian product centre
+  double aij;         // ai + aj
+  double kab;         // ci * cj * exp(-ai aj / aij |A-B|^2)
+  double cce0;
+  double ai, aj;
+};
+
+// A unit of work: one contracted (virtual) quartet in canonical shell order.
+struct Task {
+  int16_t sh[4];   // virtual shells, canonical order (sh[3] unused for 3c2e)
+  int64_t out;     // offset of element (0,0,0,0) of this quartet's block
+};
+
+// Device view of a basis (atm/bas/env) plus its pair data.
+struct BasisView {
+  const VShell* vsh;
+  const double* env;
+  const int64_t* pair_off;   // [nv * nv], offset of pair (a,b) in pp[]
+  const double* pair_ld;     // [nv * nv], log(|A-B| + 1)
+  const PairPrim* pp;
+  int nv;
+  double expcutoff_4c;  // libcint's cutoff for 4c integrals
+  double expcutoff_3c;
+  double omega;         // env[PTR_RANGE_OMEGA]
+};
+
+// Launch description of one bucket: tasks [t0, t1) share class, kind,
+// permutation and team size.
+struct BucketArgs {
+  const Task* tasks;
+  int64_t t0, t1;
+  double* out;           // output base
+  int* not0;             // per task status (nullable), indexed by task id
+  const int* task_id;    // original list position of each task (for not0), nullable
+  double* scratch;       // generic path: per-task accumulators
+  int64_t scratch_stride;
+  int la, lb, lc, ld;    // canonical angular momenta
+  int kind;
+  int sph;
+  int perm;              // canonical axis A maps to original axis (perm >> 2A) & 3
+  int team;              // lanes per task (fast kernels)
+};
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
+
+}  // namespace mcint

@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every symbol include/mcint.h declares
+(no GPU needed: nothing here calls into HIP)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    txt = open(os.path.join(ROOT, "include", "mcint.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:CACHE_SIZE_T|FINT|int|void|double|const char \*)\s*\*?\s*(\w+)\s*\(",
+                       txt, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def libpath():
+    import libcint_amd
+    if not os.path.exists(libcint_amd.LIB_PATH):
+        from libcint_amd import build
+        build.build()
+    return libcint_amd.LIB_PATH
+
+
+def test_header_declares_reference_entry_points():
+    names = _declared_functions()
+    for n in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
+              "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "CINTdel_optimizer",
+              "CINTdel_2e_optimizer", "cint2e_sph", "cint2e_cart", "cint2e_sph_optimizer",
+              "cint2e_ip1_sph", "CINTgto_norm", "CINTcgto_spheric", "CINTcgto_cart",
+              "CINTtot_cgto_spheric", "CINTshells_spheric_offset", "CINTrys_roots",
+              "int2e_sph_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch"):
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol(libpath):
+    L = ctypes.CDLL(libpath)
+    missing = [n for n in _declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", libpath], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    assert not [n for n in _declared_functions() if n not in exported]
+
+
+def test_host_helpers_match_libcint_conventions(libpath):
+    import numpy as np
+    import libcint_amd
+    from libcint_amd.synthetic import gto_norm
+    L = libcint_amd.lib()
+    for l in range(5):
+        for a in (0.1, 1.3, 27.0):
+            assert abs(L.CINTgto_norm(l, a) - gto_norm(l, a)) <= 1e-14 * gto_norm(l, a)
+    bas = np.array([[0, 2, 1, 2, 0, 20, 21, 0], [0, 1, 1, 1, 0, 20, 21, 0]], dtype=np.int32)
+    Lb = L
+    Lb.CINTtot_cgto_spheric.restype = ctypes.c_int
+    assert Lb.CINTtot_cgto_spheric(bas.ctypes.data_as(ctypes.c_void_p), 2) == 10 + 3
+    ao = np.zeros(2, dtype=np.int32)
+    Lb.CINTshells_spheric_offset(ao.ctypes.data_as(ctypes.c_void_p), bas.ctypes.data_as(ctypes.c_void_p), 2)
+    assert list(ao) == [0, 10]
+
+
+def test_python_package_fails_loudly_without_library(monkeypatch, tmp_path):
+    import libcint_amd
+    monkeypatch.setattr(libcint_amd, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(libcint_amd, "_lib", None)
+    with pytest.raises(libcint_amd.McintError):
+        libcint_amd.lib()

@@ -1,0 +1,217 @@
+"""GPU parity: libmcint (HIP, through the C ABI) against the CPU oracle.
+
+Bar: per element |gpu - ref| <= 1e-12 + 1e-10 |ref| (FP64 path), the
+checksums of libcint's own test suite, and device Rys roots within 1e-13
+relative of the exact quad-precision rule.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import close_place, libcint_quartets
+
+pytestmark = pytest.mark.gpu
+
+ATOL, RTOL = 1e-12, 1e-10
+
+
+def _say(*a):
+    print("[gpu-parity]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mc():
+    import libcint_amd
+    libcint_amd.lib()
+    return libcint_amd
+
+
+def _cmp(got, ref, what):
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    err = np.abs(got - ref)
+    bad = err > ATOL + RTOL * np.abs(ref)
+    if bad.any():
+        i = int(np.argmax(err - (ATOL + RTOL * np.abs(ref))))
+        raise AssertionError(f"{what}: {int(bad.sum())} elements off, worst at {i}: "
+                             f"got {got.ravel()[i]!r} ref {ref.ravel()[i]!r}")
+
+
+def test_device_rys_roots(mc, orc):
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([[0.0, 1e-9], rng.uniform(0, 2, 60), rng.uniform(0, 110, 200),
+                         np.arange(0.0, 100.0, 1.0), [150.0, 1e4]])
+    for n in range(1, 10):
+        _say("roots n =", n)
+        u, w = mc.rys_roots(n, xs)
+        for i, x in enumerate(xs):
+            u0, w0 = orc.rys_roots(n, float(x), exact=True)
+            np.testing.assert_allclose(u[i], u0, rtol=1e-13, atol=0, err_msg=f"u n={n} x={x}")
+            np.testing.assert_allclose(w[i], w0, rtol=1e-13, atol=0, err_msg=f"w n={n} x={x}")
+
+
+@pytest.mark.parametrize("name,key", [("int2e_sph", "cint2e_sph"), ("int2e_cart", None),
+                                      ("int2e_ip1_sph", "cint2e_ip1_sph")])
+def test_libcint_fixture(mc, orc, golden, fixture_basis, name, key):
+    """All (i<=j, k<=l) quartets of testsuite/test_cint.py's fixture: l <= 4,
+    a 2-primitive 2-contraction d shell, up to 9 Rys roots."""
+    _say("fixture", name)
+    q = libcint_quartets()
+    B = mc.Basis(*fixture_basis)
+    got, off, not0 = mc.batch(name, q, B)
+    ref, off_r, not0_r = orc.batch(name, q, orc.Basis(*fixture_basis))
+    np.testing.assert_array_equal(off, off_r)
+    _cmp(got, ref, name)
+    np.testing.assert_array_equal(not0, not0_r)
+    if key:
+        v = float(np.abs(got).sum())
+        gref, place = golden["golden"][key]
+        assert close_place(v, gref, got.size, place), (v, gref)
+
+
+def test_libcint_fixture_3c2e(mc, orc, golden, fixture_basis):
+    _say("fixture int3c2e_sph")
+    q = np.array([(i, j, k, 0) for k in range(4) for j in range(4) for i in range(4)], np.int32)
+    got, _, _ = mc.batch("int3c2e_sph", q, mc.Basis(*fixture_basis))
+    ref, _, _ = orc.batch("int3c2e_sph", q, orc.Basis(*fixture_basis))
+    _cmp(got, ref, "int3c2e_sph")
+    v = float(np.abs(got).sum())
+    gref, place = golden["golden"]["cint3c2e_sph"]
+    assert close_place(v, gref, got.size, place), (v, gref)
+
+
+def _sample_quartets(nbas, n, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, nbas, size=(n, 4)).astype(np.int32)
+    return q
+
+
+@pytest.mark.parametrize("cfg,n", [("c0", 3000), ("c1", 1200), ("c3", 800)])
+def test_synthetic_samples(mc, orc, cfg, n):
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule(cfg)
+    nb = m.nbas if m.aux_start < 0 else m.aux_start
+    q = _sample_quartets(nb, n, seed=hash(cfg) % 1000)
+    _say(cfg, "sample of", n, "quartets")
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        for name in ("int2e_sph", "int2e_ip1_sph"):
+            qq = q if name == "int2e_sph" else q[: n // 4]
+            got, _, nz = mc.batch(name, qq, mc.Basis(m.atm, m.bas, m.env))
+            ref, _, nz_r = orc.batch(name, qq, orc.Basis(m.atm, m.bas, m.env))
+            _cmp(got, ref, f"{cfg} {name}")
+            np.testing.assert_array_equal(nz, nz_r)
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)
+
+
+def test_c4_3c2e_sample(mc, orc):
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c4")
+    rng = np.random.default_rng(4)
+    n = 600
+    q = np.zeros((n, 4), np.int32)
+    q[:, 0] = rng.integers(0, m.aux_start, n)
+    q[:, 1] = rng.integers(0, m.aux_start, n)
+    q[:, 2] = rng.integers(m.aux_start, m.nbas, n)
+    _say("c4 int3c2e sample")
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        got, _, _ = mc.batch("int3c2e_sph", q, mc.Basis(m.atm, m.bas, m.env))
+        ref, _, _ = orc.batch("int3c2e_sph", q, orc.Basis(m.atm, m.bas, m.env))
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)
+    _cmp(got, ref, "c4 int3c2e_sph")
+
+
+def test_per_call_with_dims(mc, orc, fixture_basis):
+    """int2e_sph(out, dims, ...) writes into a larger block (src/cint2e.c:853-856)."""
+    import ctypes
+    _say("per-call dims")
+    L = mc.lib()
+    atm, bas, env = fixture_basis
+    B = mc.Basis(atm, bas, env)
+    shls = np.array([1, 2, 0, 5], np.int32)
+    d = [B.shell_dim(int(s)) for s in shls]
+    dims = np.array([d[0] + 2, d[1] + 1, d[2] + 3, d[3]], np.int32)
+    out = np.full(int(np.prod(dims)), 7.0)
+    rc = L.int2e_sph(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                     dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                     shls.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                     B.atm.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B.natm,
+                     B.bas.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B.nbas,
+                     B.env.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)
+    ref, rc_r = orc.intor("int2e_sph", tuple(shls), orc.Basis(atm, bas, env))
+    assert rc == rc_r == 1
+    o = out.reshape(dims, order="F")
+    _cmp(o[: d[0], : d[1], : d[2], : d[3]], ref, "dims block")
+    assert np.all(o[d[0]:] == 7.0) and np.all(o[:, d[1]:] == 7.0)
+
+
+def test_screened_quartet_is_zero_filled(mc, orc):
+    """Far-apart tight primitives: every primitive pair screened -> zeros, not0 = 0."""
+    env = [0.0] * 20
+    atm, bas = [], []
+    for r in ((0, 0, 0), (40.0, 0, 0)):
+        atm.append([1, len(env), 1, 0, 0, 0])
+        env.extend(r)
+    for ia in (0, 1):
+        bas.append([ia, 1, 1, 1, 0, len(env), len(env) + 1, 0])
+        env.extend([3.0, 1.0])
+    B = (np.array(atm, np.int32), np.array(bas, np.int32), np.array(env))
+    q = np.array([[0, 1, 0, 1], [0, 0, 1, 1]], np.int32)
+    got, _, nz = mc.batch("int2e_sph", q, mc.Basis(*B))
+    ref, _, nz_r = orc.batch("int2e_sph", q, orc.Basis(*B))
+    np.testing.assert_array_equal(nz, nz_r)
+    assert nz[0] == 0 and nz[1] == 1
+    _cmp(got, ref, "screened")
+    assert np.all(got[:81] == 0)
+
+
+def test_plan_graph_replay_matches_direct(mc):
+    """Plan outputs via HIP-graph replays (with eager work and syncs between)
+    equal the directly launched plan, on every replay."""
+    import torch
+    from libcint_amd.synthetic import make_molecule, unique_quartets
+    m = make_molecule("c0")
+    q = unique_quartets(m.nbas)[::97].copy()
+    _say("plan graph replay", len(q), "quartets")
+    db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, m.env))
+    plan = mc.Plan(db, "int2e_sph", q)
+    dev = torch.device("cuda:0")
+    ref = torch.zeros(plan.out_size, dtype=torch.float64, device=dev)
+    plan.execute(ref.data_ptr(), graph=False)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
+    for rep in range(3):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.execute(out.data_ptr(), stream_ptr=s.cuda_stream, graph=True)
+        s.synchronize()
+        junk = torch.randn(1000, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), f"replay {rep}"
+    del junk
+
+
+def test_schwarz_bounds(mc, orc):
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c0")
+    db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, m.env))
+    _say("schwarz")
+    qb = db.schwarz()
+    B = orc.Basis(m.atm, m.bas, m.env)
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        for i, j in ((0, 0), (5, 3), (47, 12), (30, 30), (47, 46)):
+            v, _ = orc.intor("int2e_sph", (i, j, i, j), B)
+            ref = np.sqrt(np.abs(v).max())
+            assert abs(qb[i, j] - ref) <= 1e-12 * ref
+            assert qb[j, i] == qb[i, j]
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)

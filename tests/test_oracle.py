@@ -1,0 +1,228 @@
+"""Pins the CPU oracle (oracle/cint_oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible mistake (a dropped term, a wrong
+sign, index or normalisation, a transposed operand) fails at least one:
+  * libcint's own golden values: checksums of int2e_sph / int2e_ip1_sph /
+    int3c2e_sph over the test-suite fixture, Boys constants, root
+    fingerprints (tests/golden/libcint_testsuite.json);
+  * mathematics: Gauss-rule moment identities against mpmath's incomplete
+    gamma, orthonormality of the solid harmonics, the (ss|ss) closed form,
+    permutational symmetry, finite differences for the i-derivative, and
+    the (ij|k) = (ij|k s0) identity of the 3-centre integral.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from conftest import close_place, libcint_quartets
+
+
+def fp(a):
+    a = np.asarray(a)
+    return a.ravel().dot(np.cos(np.arange(a.size)))
+
+
+def mp_boys(k, x):
+    x = mp.mpf(x)
+    if x == 0:
+        return mp.mpf(1) / (2 * k + 1)
+    return mp.gammainc(k + mp.mpf(1) / 2, 0, x) / (2 * x ** (k + mp.mpf(1) / 2))
+
+
+def test_boys_constants(orc, golden):
+    # testsuite/test_rys_roots.py:64-65
+    np.testing.assert_allclose(orc.boys(2, 1.0), golden["golden"]["boys_x1"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(orc.boys(2, 7.0), golden["golden"]["boys_x7"], rtol=0, atol=1e-15)
+
+
+def test_boys_vs_mpmath(orc):
+    mp.mp.dps = 30
+    for x in (0.0, 1e-12, 0.3, 5.0, 29.9, 60.0, 150.0, 250.0):
+        f = orc.boys(18, x)
+        for k in range(19):
+            ref = float(mp_boys(k, x))
+            assert abs(f[k] - ref) <= 2e-16 * ref, (x, k)
+
+
+def test_rys_fingerprints(orc, golden):
+    # testsuite/test_rys_roots.py:72-85 (the reference generated these with
+    # a 15-digit mpmath run; the n=6 pair carries that run's ~1e-10 error)
+    for n, x, ref in golden["golden"]["fp_rys"]:
+        u, w = orc.rys_roots(n, x)
+        assert abs(fp(np.array([u, w])) - ref) < 1e-12 * max(1.0, abs(ref)) * 10, (n, x)
+    x, fr, fw = golden["golden"]["fp_rys6"]
+    u, w = orc.rys_roots(6, x)
+    assert abs(fp(u) - fr) < 1e-9
+    assert abs(fp(w) - fw) < 1e-9
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_rys_moments(orc, exact):
+    """n-point Gauss rule of exp(-x t^2) on [0,1]: sum w s^k = F_k(x), k < 2n."""
+    mp.mp.dps = 30
+    rng = np.random.default_rng(7)
+    xs = [0.0, 1e-7, 0.37, 2.5, 11.2, 33.3, 47.9, 70.1, 95.0, 300.0] + list(rng.uniform(0, 100, 6))
+    for n in range(1, 10):
+        for x in xs:
+            u, w = orc.rys_roots(n, x, exact=exact)
+            s = u / (1 + u)
+            assert np.all(w > 0) and np.all(np.diff(s) > 0)
+            for k in range(2 * n):
+                ref = float(mp_boys(k, x))
+                got = float(np.sum(w * s ** k))
+                assert abs(got - ref) <= 2e-13 * ref, (n, x, k, got, ref)
+
+
+def test_rys_table_vs_exact(orc):
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.uniform(0, 1, 40), rng.uniform(0, 100, 160), np.arange(0, 95, 1.0)])
+    for n in range(1, 10):
+        for x in xs:
+            u0, w0 = orc.rys_roots(n, x, exact=True)
+            u1, w1 = orc.rys_roots(n, x, exact=False)
+            np.testing.assert_allclose(u1, u0, rtol=2e-14, atol=0)
+            np.testing.assert_allclose(w1, w0, rtol=2e-14, atol=0)
+
+
+def _sphere_monomial(a, b, c):
+    if a % 2 or b % 2 or c % 2:
+        return 0.0
+    g = math.gamma
+    return 2 * g((a + 1) / 2) * g((b + 1) / 2) * g((c + 1) / 2) / g((a + b + c + 3) / 2)
+
+
+def test_c2s_orthonormal(orc):
+    for l in range(2, 7):
+        C = orc.c2s_matrix(l)
+        comps = [(lx, ly, l - lx - ly) for lx in range(l, -1, -1) for ly in range(l - lx, -1, -1)]
+        nc = len(comps)
+        S = np.array([[_sphere_monomial(*(np.add(comps[i], comps[j]))) for j in range(nc)]
+                      for i in range(nc)])
+        np.testing.assert_allclose(C @ S @ C.T, np.eye(2 * l + 1), atol=1e-13)
+    # d shell values of libcint's table (src/cart2sph.c:49-80)
+    d = orc.c2s_matrix(2)
+    assert abs(d[0, 1] - 1.092548430592079070) < 1e-15
+    assert abs(d[2, 5] - 0.630783130505040012) < 1e-15
+    assert abs(d[2, 0] + 0.315391565252520002) < 1e-15
+    assert abs(d[4, 3] + 0.546274215296039535) < 1e-15
+    np.testing.assert_array_equal(orc.c2s_matrix(1), np.eye(3))
+
+
+@pytest.mark.parametrize("mode", ["table", "exact"])
+def test_int2e_checksums(orc, golden, fixture_basis, mode):
+    """testsuite/test_cint.py:478-481 over all 8 shells (l <= 4, nroots <= 9)."""
+    orc.set_root_mode(orc.ROOTS_EXACT if mode == "exact" else orc.ROOTS_TABLE)
+    try:
+        B = orc.Basis(*fixture_basis)
+        q = libcint_quartets()
+        for name, key in (("int2e_sph", "cint2e_sph"), ("int2e_ip1_sph", "cint2e_ip1_sph")):
+            out, off, not0 = orc.batch(name, q, B)
+            v = float(np.abs(out).sum())
+            ref, place = golden["golden"][key]
+            assert close_place(v, ref, out.size, place), (name, v, ref)
+            assert abs(v - ref) < 1e-12 * ref, (name, v, ref)
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)
+
+
+def test_int3c2e_checksum_and_s0_identity(orc, golden, fixture_basis):
+    """testsuite/test_3c2e.py:170-202 and :303-307."""
+    atm, bas, env = fixture_basis
+    bas = bas.copy()
+    v, cnt = 0.0, 0
+    for k in range(4):
+        bas[8, 0] = bas[k, 0]  # s0 shell on the atom of k
+        B = orc.Basis(atm, bas, env)
+        for j in range(4):
+            for i in range(4):
+                a, _ = orc.intor("int3c2e_sph", (i, j, k), B)
+                ref, _ = orc.intor("int2e_sph", (i, j, k, 8), B)
+                tol = 1e-12 * max(1.0, np.abs(ref).max())
+                np.testing.assert_allclose(a.ravel(order="F"), ref.ravel(order="F"), rtol=1e-11,
+                                           atol=tol)
+                v += float(np.abs(a).sum())
+                cnt += a.size
+    gref, place = golden["golden"]["cint3c2e_sph"]
+    assert close_place(v, gref, cnt, place), (v, gref)
+
+
+def _single_s_basis(exps, coefs, centres):
+    atm, bas, env = [], [], [0.0] * 20
+    for a, c, r in zip(exps, coefs, centres):
+        atm.append([1, len(env), 1, 0, 0, 0])
+        env.extend(r)
+        bas.append([len(atm) - 1, 0, 1, 1, 0, len(env), len(env) + 1, 0])
+        env.extend([a, c])
+    return np.array(atm, np.int32), np.array(bas, np.int32), np.array(env)
+
+
+def test_ssss_closed_form(orc):
+    """(ss|ss) = 2 pi^(5/2) / (p q sqrt(p+q)) K_AB K_CD F_0(T) on raw Gaussians."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        ex = rng.uniform(0.1, 5.0, 4)
+        cf = rng.uniform(0.5, 2.0, 4)
+        cen = rng.uniform(-2, 2, (4, 3))
+        B = orc.Basis(*_single_s_basis(ex, cf, cen))
+        val, _ = orc.intor("int2e_sph", (0, 1, 2, 3), B)
+        a, b, c, d = ex
+        p, q = a + b, c + d
+        P = (a * cen[0] + b * cen[1]) / p
+        Q = (c * cen[2] + d * cen[3]) / q
+        kab = math.exp(-a * b / p * np.sum((cen[0] - cen[1]) ** 2))
+        kcd = math.exp(-c * d / q * np.sum((cen[2] - cen[3]) ** 2))
+        T = p * q / (p + q) * np.sum((P - Q) ** 2)
+        f0 = float(mp_boys(0, T))
+        ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * kab * kcd * f0
+        ref *= np.prod(cf) * (0.5 / math.sqrt(math.pi)) ** 4
+        assert abs(val.item() - ref) <= 1e-14 * abs(ref), (val.item(), ref)
+
+
+def test_permutational_symmetry(orc):
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c0")
+    B = orc.Basis(m.atm, m.bas, m.env)
+    rng = np.random.default_rng(5)
+    for _ in range(25):
+        i, j, k, l = rng.integers(0, m.nbas, 4)
+        a, _ = orc.intor("int2e_sph", (i, j, k, l), B)
+        b, _ = orc.intor("int2e_sph", (j, i, k, l), B)
+        c, _ = orc.intor("int2e_sph", (k, l, i, j), B)
+        d, _ = orc.intor("int2e_sph", (i, j, l, k), B)
+        np.testing.assert_allclose(b.transpose(1, 0, 2, 3), a, rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(c.transpose(2, 3, 0, 1), a, rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(d.transpose(0, 1, 3, 2), a, rtol=1e-11, atol=1e-14)
+
+
+def test_ip1_finite_difference(orc):
+    """int2e_ip1 = <d/dr i> = -d/dA_i of int2e_cart (shell i alone on its atom)."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c3")
+    # give shell i its own atom by appending a ghost atom at the same place
+    rng = np.random.default_rng(9)
+    h = 1e-4
+    done = 0
+    while done < 6:
+        i, j, k, l = (int(v) for v in rng.integers(0, m.nbas, 4))
+        if i in (j, k, l):
+            continue  # only shell i may move
+        done += 1
+        atm = np.vstack([m.atm, m.atm[m.bas[i, 0]]]).astype(np.int32)
+        env = np.concatenate([m.env, m.env[m.atm[m.bas[i, 0], 1]:m.atm[m.bas[i, 0], 1] + 3]])
+        atm[-1, 1] = len(m.env)
+        bas = m.bas.copy()
+        bas[i, 0] = len(atm) - 1
+        g, _ = orc.intor("int2e_ip1_cart", (i, j, k, l), orc.Basis(atm, bas, env))
+        for d in range(3):
+            ep, em = env.copy(), env.copy()
+            ep[len(m.env) + d] += h
+            em[len(m.env) + d] -= h
+            bp = orc.Basis(atm, bas, ep)
+            bm = orc.Basis(atm, bas, em)
+            # shells j,k,l may share the original atom: they stay put
+            vp, _ = orc.intor("int2e_cart", (i, j, k, l), bp)
+            vm, _ = orc.intor("int2e_cart", (i, j, k, l), bm)
+            fd = -(vp - vm) / (2 * h)
+            np.testing.assert_allclose(g[..., d], fd, rtol=0, atol=2e-7 * max(1.0, np.abs(fd).max()))
