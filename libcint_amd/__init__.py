@@ -43,6 +43,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise McintError(f"{LIB_PATH} is missing: run `python -m libcint_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname, libamdhip64.so.7).  Loading torch first makes libmcint bind to
+    # that copy instead of pulling in a second runtime from /opt/rocm.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
