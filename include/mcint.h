@@ -174,6 +174,16 @@ void mcint_plan_destroy(mcint_plan *p);
  * screening, n[5] output doubles. */
 int mcint_plan_stats(const mcint_plan *p, int64_t *n);
 
+/* Buckets of a plan (one kernel launch each).  info receives: la, lb, lc,
+ * ld (canonical class), perm, team lanes, fast(1)/generic(0), task count. */
+int mcint_plan_nbuckets(const mcint_plan *p);
+int mcint_plan_bucket_info(const mcint_plan *p, int bucket, int64_t *info);
+int mcint_plan_execute_bucket(mcint_plan *p, int bucket, double *out_dev, FINT *not0_dev,
+                              void *hip_stream);
+/* Primitive quartets that survive libcint's screening, per bucket (host
+ * array of nbuckets counts).  Synchronous. */
+int mcint_plan_count_primitives(mcint_plan *p, int64_t *per_bucket);
+
 /* Schwarz bounds q[i*nbas+j] = sqrt(max |(ij|ij)|) over the spherical
  * block of every shell pair among the first nshell shells (nshell <= 0:
  * all).  q is a host array of nbas*nbas doubles. */

@@ -77,6 +77,14 @@ def lib():
     L.mcint_plan_destroy.argtypes = [vp]
     L.mcint_plan_stats.restype = ctypes.c_int
     L.mcint_plan_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_nbuckets.restype = ctypes.c_int
+    L.mcint_plan_nbuckets.argtypes = [vp]
+    L.mcint_plan_bucket_info.restype = ctypes.c_int
+    L.mcint_plan_bucket_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_execute_bucket.restype = ctypes.c_int
+    L.mcint_plan_execute_bucket.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    L.mcint_plan_count_primitives.restype = ctypes.c_int
+    L.mcint_plan_count_primitives.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.mcint_schwarz.restype = ctypes.c_int
     L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
     L.mcint_last_error.restype = ctypes.c_char_p
@@ -228,6 +236,29 @@ class Plan:
         _check(lib().mcint_plan_stats(self.handle, n), "mcint_plan_stats")
         return dict(tasks=n[0], buckets=n[1], fast_tasks=n[2], generic_tasks=n[3],
                     prim_quartets=n[4], out_size=n[5])
+
+    def buckets(self) -> list:
+        """Per bucket: dict(cls=(la,lb,lc,ld), perm, team, fast, tasks)."""
+        L = lib()
+        out = []
+        info = (ctypes.c_int64 * 8)()
+        for i in range(L.mcint_plan_nbuckets(self.handle)):
+            _check(L.mcint_plan_bucket_info(self.handle, i, info), "mcint_plan_bucket_info")
+            out.append(dict(cls=tuple(info[:4]), perm=info[4], team=info[5], fast=bool(info[6]),
+                            tasks=info[7]))
+        return out
+
+    def count_primitives(self) -> np.ndarray:
+        n = lib().mcint_plan_nbuckets(self.handle)
+        c = np.zeros(max(n, 1), dtype=np.int64)
+        _check(lib().mcint_plan_count_primitives(
+            self.handle, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))), "count_primitives")
+        return c[:n]
+
+    def execute_bucket(self, i: int, out_ptr: int, stream_ptr: int = 0):
+        _check(lib().mcint_plan_execute_bucket(self.handle, i, ctypes.c_void_p(out_ptr), None,
+                                               ctypes.c_void_p(stream_ptr or None)),
+               "mcint_plan_execute_bucket")
 
     def execute(self, out_ptr: int, not0_ptr: int = 0, stream_ptr: int = 0, graph: bool = True):
         """Run on device pointers (e.g. torch tensor .data_ptr(), stream.cuda_stream)."""

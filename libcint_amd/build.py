@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmcint.so")
 
-SOURCES = ["eri.hip", "api.hip"]
+SOURCES = ["eri.hip", "api.hip"] + [f"fast_{g}.hip" for g in range(9)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCINT_ARCH", "gfx950")
@@ -32,6 +32,8 @@ CFLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
     "-Wno-unused-result",
 ]
+if os.environ.get("MCINT_RESOURCE_USAGE"):
+    CFLAGS += ["-Rpass-analysis=kernel-resource-usage"]
 
 
 def _compile(src: str) -> str:
@@ -42,6 +44,9 @@ def _compile(src: str) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     print(f"  compiled {src} in {time.time() - t:.1f}s", file=sys.stderr)
+    if os.environ.get("MCINT_RESOURCE_USAGE"):
+        with open(obj + ".remarks.txt", "w") as fh:
+            fh.write(r.stderr)
     return obj
 
 

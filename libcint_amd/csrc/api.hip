@@ -500,8 +500,8 @@ extern "C" int mcint_plan_stats(const mcint_plan* p, int64_t* n) {
   return MCINT_OK;
 }
 
-static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream) {
-  mcint_basis* b = p->b;
+static BasisView plan_view(const mcint_plan* p) {
+  const mcint_basis* b = p->b;
   BasisView B;
   B.vsh = b->vsh_d;
   B.env = b->env_d;
@@ -512,35 +512,99 @@ static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t strea
   B.expcutoff_4c = b->expc4;
   B.expcutoff_3c = b->expc3;
   B.omega = b->omega;
-  for (const Bucket& bk : p->buckets) {
-    BucketArgs A;
-    A.tasks = p->tasks_d;
-    A.out = out;
-    A.not0 = not0;
-    A.task_id = p->task_id_d;
-    A.scratch = p->scratch_d;
-    A.scratch_stride = bk.sstride;
-    A.la = bk.la;
-    A.lb = bk.lb;
-    A.lc = bk.lc;
-    A.ld = bk.ld;
-    A.kind = p->kind;
-    A.sph = p->sph;
-    A.perm = bk.perm;
-    A.team = bk.team;
-    if (bk.fast) {
-      A.t0 = bk.t0;
-      A.t1 = bk.t1;
-      HIPCHK(launch_fast(B, A, stream));
-    } else {
-      const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
-      for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
-        A.t0 = t;
-        A.t1 = std::min(bk.t1, t + chunk);
-        HIPCHK(launch_generic(B, A, stream));
-      }
+  return B;
+}
+
+static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out, FINT* not0) {
+  BucketArgs A;
+  A.tasks = p->tasks_d;
+  A.out = out;
+  A.not0 = not0;
+  A.task_id = p->task_id_d;
+  A.scratch = p->scratch_d;
+  A.scratch_stride = bk.sstride;
+  A.la = bk.la;
+  A.lb = bk.lb;
+  A.lc = bk.lc;
+  A.ld = bk.ld;
+  A.kind = p->kind;
+  A.sph = p->sph;
+  A.perm = bk.perm;
+  A.team = bk.team;
+  A.t0 = bk.t0;
+  A.t1 = bk.t1;
+  return A;
+}
+
+static int bucket_launch(mcint_plan* p, const Bucket& bk, double* out, FINT* not0,
+                         hipStream_t stream) {
+  const BasisView B = plan_view(p);
+  BucketArgs A = bucket_args(p, bk, out, not0);
+  if (bk.fast) {
+    HIPCHK(launch_fast(B, A, stream));
+  } else {
+    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
+    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+      A.t0 = t;
+      A.t1 = std::min(bk.t1, t + chunk);
+      HIPCHK(launch_generic(B, A, stream));
     }
   }
+  return MCINT_OK;
+}
+
+static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream) {
+  for (const Bucket& bk : p->buckets) {
+    const int st = bucket_launch(p, bk, out, not0, stream);
+    if (st) return st;
+  }
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_nbuckets(const mcint_plan* p) { return p ? (int)p->buckets.size() : 0; }
+
+extern "C" int mcint_plan_bucket_info(const mcint_plan* p, int i, int64_t* info) {
+  if (!p || !info || i < 0 || i >= (int)p->buckets.size()) {
+    set_err("mcint_plan_bucket_info: bad bucket index");
+    return MCINT_ERR_ARG;
+  }
+  const Bucket& b = p->buckets[i];
+  const int64_t v[8] = {b.la, b.lb, b.lc, b.ld, b.perm, b.team, b.fast ? 1 : 0, b.t1 - b.t0};
+  for (int k = 0; k < 8; ++k) info[k] = v[k];
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_execute_bucket(mcint_plan* p, int i, double* out_dev, FINT* not0_dev,
+                                         void* hip_stream) {
+  if (!p || i < 0 || i >= (int)p->buckets.size() || !out_dev) {
+    set_err("mcint_plan_execute_bucket: bad argument");
+    return MCINT_ERR_ARG;
+  }
+  return bucket_launch(p, p->buckets[i], out_dev, not0_dev, (hipStream_t)hip_stream);
+}
+
+extern "C" int mcint_plan_count_primitives(mcint_plan* p, int64_t* per_bucket) {
+  if (!p || !per_bucket) {
+    set_err("mcint_plan_count_primitives: null argument");
+    return MCINT_ERR_ARG;
+  }
+  const size_t nb = p->buckets.size();
+  unsigned long long* cnt = nullptr;
+  HIPCHK(dalloc(&cnt, std::max<size_t>(nb, 1)));
+  hipError_t e = hipMemset(cnt, 0, std::max<size_t>(nb, 1) * sizeof(unsigned long long));
+  const BasisView B = plan_view(p);
+  for (size_t i = 0; i < nb && e == hipSuccess; ++i) {
+    BucketArgs A = bucket_args(p, p->buckets[i], nullptr, nullptr);
+    e = launch_count_prims(B, A, cnt + i, nullptr);
+  }
+  std::vector<unsigned long long> h(std::max<size_t>(nb, 1));
+  if (e == hipSuccess) e = hipMemcpy(h.data(), cnt, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  dfree(cnt);
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_plan_count_primitives: ") + hipGetErrorString(e));
+    return MCINT_ERR_HIP;
+  }
+  for (size_t i = 0; i < nb; ++i) per_bucket[i] = (int64_t)h[i];
   return MCINT_OK;
 }
 

@@ -1,0 +1,24 @@
+// Device-side helpers shared by every device translation unit.
+#pragma once
+#include "eri_launch.h"
+#include "rys_device.h"
+
+#define C2S_QUAL static __device__ const
+#include "c2s_tables.inc"
+#undef C2S_QUAL
+
+namespace mcint {
+
+__device__ __forceinline__ double common_fac_sp(int l) {
+  // 1/(2 sqrt(pi)) and sqrt(3/(4 pi)); src/g1e.c:565-572
+  return l == 0 ? 0.282094791773878143 : (l == 1 ? 0.488602511902919921 : 1.0);
+}
+
+constexpr double PI3_2_SQRTPI = 34.986836655249725;  // 2 pi^3 / sqrt(pi)
+
+__device__ __forceinline__ int cart_idx(int l, int lx, int lz) {
+  const int m = l - lx;
+  return m * (m + 1) / 2 + lz;
+}
+
+}  // namespace mcint

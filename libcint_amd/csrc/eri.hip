@@ -11,26 +11,9 @@
 //   |c,d) <- |f0] done once on contracted data (libcint does them per
 //   primitive, src/g2e.c:428-693), the Cartesian -> spherical transform
 //   (src/cart2sph.c:5324-5376) and the store in libcint's Fortran order.
-#include "eri_launch.h"
-#include "rys_device.h"
-
-#define C2S_QUAL static __device__ const
-#include "c2s_tables.inc"
-#undef C2S_QUAL
+#include "device_common.h"
 
 namespace mcint {
-
-__device__ __forceinline__ double common_fac_sp(int l) {
-  // 1/(2 sqrt(pi)) and sqrt(3/(4 pi)); src/g1e.c:565-572
-  return l == 0 ? 0.282094791773878143 : (l == 1 ? 0.488602511902919921 : 1.0);
-}
-
-constexpr double PI3_2_SQRTPI = 34.986836655249725;  // 2 pi^3 / sqrt(pi)
-
-__device__ __forceinline__ int cart_idx(int l, int lx, int lz) {
-  const int m = l - lx;
-  return m * (m + 1) / 2 + lz;
-}
 
 // ---------------------------------------------------------------------
 // Pair data: one block per ordered virtual-shell pair, one lane per
@@ -516,6 +499,48 @@ hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, do
   return hipGetLastError();
 }
 
+// Counts primitive quartets that pass libcint's screening (one lane per
+// task); used by the benchmark's algorithmic FLOP model.
+__global__ void count_prims_kernel(BasisView B, BucketArgs A, unsigned long long* counter) {
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.t1) return;
+  const Task tk = A.tasks[t];
+  const bool ip1 = A.kind == KIND_2E_IP1;
+  const bool three = A.kind == KIND_3C2E;
+  const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
+  const int nij = s0.nprim * s1.nprim;
+  const int nkl = three ? s2.nprim : s2.nprim * B.vsh[tk.sh[3]].nprim;
+  const PairPrim* pij = B.pp + B.pair_off[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
+  const PairPrim* pkl = three ? nullptr : B.pp + B.pair_off[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
+  const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]] * (A.la + A.lb + (ip1 ? 1 : 0));
+  const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]] * (A.lc + A.ld);
+  const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
+  int live_ij = 0, live_kl = three;
+  for (int p = 0; p < nij; ++p) live_ij |= (pij[p].cce0 - ld_ij) < E;
+  if (!three)
+    for (int p = 0; p < nkl; ++p) live_kl |= (pkl[p].cce0 - ld_kl) < E;
+  if (!(live_ij && live_kl)) return;
+  unsigned long long n = 0;
+  for (int kl = 0; kl < nkl; ++kl) {
+    const double ckl = three ? 0.0 : pkl[kl].cce0 - ld_kl;
+    if (ckl >= E) continue;
+    for (int ij = 0; ij < nij; ++ij) {
+      const double cij = pij[ij].cce0 - ld_ij;
+      if (cij < E && cij <= (three ? E : E - ckl)) ++n;
+    }
+  }
+  atomicAdd(counter, n);
+}
+
+hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned long long* counter,
+                              hipStream_t stream) {
+  const int64_t n = A.t1 - A.t0;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(count_prims_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, B,
+                     A, counter);
+  return hipGetLastError();
+}
+
 hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream) {
   const int64_t n = A.t1 - A.t0;
   if (n <= 0) return hipSuccess;
@@ -527,3 +552,7 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
 }  // namespace mcint
 
 #include "eri_fast.h"
+
+namespace mcint {
+MCINT_FAST_DISPATCH
+}  // namespace mcint

@@ -1,12 +1,411 @@
-// Register-resident Rys kernels (placeholder: no class has a fast kernel yet).
+// Register-resident Rys kernels for the classes whose primitive-contracted
+// [e0|f0] block fits in registers (templated on the canonical class).
+//
+// Lanes -> work: a team of T lanes (T a power of two, chosen per bucket from
+// the primitive-quartet count) owns one contracted quartet; lane t of the
+// team takes primitive quartets t, t+T, ...  Per primitive quartet a lane
+// evaluates the NR Rys roots, runs the 2D recurrence root by root in
+// registers and accumulates [e0|f0] (e on centre A, f on centre C) into a
+// register array.  The team then sums its lanes with cross-lane shuffles and
+// every lane runs the contracted-data epilogue (horizontal transfers,
+// Cartesian->spherical, store), each lane writing every T-th element.
 #pragma once
+#include <type_traits>
+#include <utility>
 
 namespace mcint {
 
-bool fast_kernel_available(int, int, int, int, int, int) { return false; }
-int fast_team_lanes_max() { return 64; }
-hipError_t launch_fast(const BasisView&, const BucketArgs&, hipStream_t) {
-  return hipErrorNotSupported;
+namespace ct {
+#define C2S_QUAL constexpr
+#include "c2s_tables.inc"
+#undef C2S_QUAL
+}  // namespace ct
+
+// ---- compile-time component bookkeeping --------------------------------
+__host__ __device__ constexpr int csum(int lo, int hi) {
+  int s = 0;
+  for (int l = lo; l <= hi; ++l) s += (l + 1) * (l + 2) / 2;
+  return s;
 }
+// shell and local index of component k of the list of shells lo, lo+1, ...
+__host__ __device__ constexpr int comp_shell(int lo, int k) {
+  int l = lo;
+  while (k >= (l + 1) * (l + 2) / 2) {
+    k -= (l + 1) * (l + 2) / 2;
+    ++l;
+  }
+  return l;
+}
+__host__ __device__ constexpr int comp_local(int lo, int k) {
+  int l = lo;
+  while (k >= (l + 1) * (l + 2) / 2) {
+    k -= (l + 1) * (l + 2) / 2;
+    ++l;
+  }
+  return k;
+}
+__host__ __device__ constexpr int local_m(int k) {
+  int m = 0;
+  while ((m + 1) * (m + 2) / 2 <= k) ++m;
+  return m;
+}
+// Cartesian exponents (libcint order: lx descending, then ly descending)
+__host__ __device__ constexpr int cpx(int lo, int k) {
+  return comp_shell(lo, k) - local_m(comp_local(lo, k));
+}
+__host__ __device__ constexpr int cpz(int lo, int k) {
+  const int kk = comp_local(lo, k);
+  const int m = local_m(kk);
+  return kk - m * (m + 1) / 2;
+}
+__host__ __device__ constexpr int cpy(int lo, int k) {
+  return comp_shell(lo, k) - cpx(lo, k) - cpz(lo, k);
+}
+// index of (x,y,z) in the list of shells lo, lo+1, ...
+__host__ __device__ constexpr int cindex(int lo, int x, int y, int z) {
+  const int l = x + y + z;
+  const int m = l - x;
+  return csum(lo, l - 1) + m * (m + 1) / 2 + z;
+}
+__host__ __device__ constexpr double binomc(int n, int k) {
+  double r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+// Cartesian -> spherical coefficient; identity for l <= 1
+__host__ __device__ constexpr double c2sc(int l, int m, int k) {
+  return l <= 1 ? (m == k ? 1.0 : 0.0) : ct::c2s_coef[ct::c2s_offset[l] + m * ((l + 1) * (l + 2) / 2) + k];
+}
+
+template <int LA, int LB, int LC, int LD>
+struct FastClass {
+  static constexpr int E = LA + LB;
+  static constexpr int F = LC + LD;
+  static constexpr int NR = (E + F) / 2 + 1;
+  static constexpr int NE = csum(LA, E);
+  static constexpr int NF = csum(LC, F);
+  static constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
+  static constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
+  static constexpr int NAo = 2 * LA + 1, NBo = 2 * LB + 1, NCo = 2 * LC + 1, NDo = 2 * LD + 1;
+};
+
+// Compile-time loop: f(std::integral_constant<int, 0..N-1>) expanded in place,
+// so every index derived from the loop variable is a constant expression.
+template <typename F, int... Is>
+__device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  sfor_impl(static_cast<F&&>(f), std::make_integer_sequence<int, N>{});
+}
+
+// 2D recurrence for one root and one Cartesian direction:
+//   V(e+1,0) = c00 V(e,0) + e b10 V(e-1,0)
+//   V(e,f+1) = c0p V(e,f) + f b01 V(e,f-1) + e b00 V(e-1,f)
+template <int E, int F>
+__device__ __forceinline__ void vrr2d(double (&V)[E + 1][F + 1], double v0, double c00, double c0p,
+                                      double b10, double b01, double b00) {
+  V[0][0] = v0;
+  sfor<E>([&](auto EI) {
+    constexpr int e = decltype(EI)::value;
+    double v = c00 * V[e][0];
+    if constexpr (e > 0) v = fma((double)e * b10, V[e - 1][0], v);
+    V[e + 1][0] = v;
+  });
+  sfor<F>([&](auto FI) {
+    constexpr int f = decltype(FI)::value;
+    sfor<E + 1>([&](auto EI) {
+      constexpr int e = decltype(EI)::value;
+      double v = c0p * V[e][f];
+      if constexpr (f > 0) v = fma((double)f * b01, V[e][f - 1], v);
+      if constexpr (e > 0) v = fma((double)e * b00, V[e - 1][f], v);
+      V[e][f + 1] = v;
+    });
+  });
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD>;
+  constexpr int E = C::E, F = C::F, NR = C::NR, NE = C::NE, NF = C::NF;
+  const int T = A.team;
+  const int tid = threadIdx.x;
+  const int tl = tid & (T - 1);
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
+  if (t >= A.t1) return;  // whole teams leave together
+
+  const Task tk = A.tasks[t];
+  const VShell& sa = Bv.vsh[tk.sh[0]];
+  const VShell& sb = Bv.vsh[tk.sh[1]];
+  const VShell& sc = Bv.vsh[tk.sh[2]];
+  const VShell& sd = Bv.vsh[tk.sh[3]];
+  const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
+  const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
+  const int nij = sa.nprim * sb.nprim;
+  const int nkl = sc.nprim * sd.nprim;
+  const int64_t pab = (int64_t)tk.sh[0] * Bv.nv + tk.sh[1];
+  const int64_t pcd = (int64_t)tk.sh[2] * Bv.nv + tk.sh[3];
+  const PairPrim* __restrict__ pij = Bv.pp + Bv.pair_off[pab];
+  const PairPrim* __restrict__ pkl = Bv.pp + Bv.pair_off[pcd];
+  const double ld_ij = Bv.pair_ld[pab] * (LA + LB);
+  const double ld_kl = Bv.pair_ld[pcd] * (LC + LD);
+  const double Ecut = Bv.expcutoff_4c;
+  // 2 pi^3 / sqrt(pi) times the s/p angular factors (src/g1e.c:565-572)
+  constexpr double cfac = 34.986836655249725 *
+                          (LA == 0 ? 0.282094791773878143 : LA == 1 ? 0.488602511902919921 : 1.0) *
+                          (LB == 0 ? 0.282094791773878143 : LB == 1 ? 0.488602511902919921 : 1.0) *
+                          (LC == 0 ? 0.282094791773878143 : LC == 1 ? 0.488602511902919921 : 1.0) *
+                          (LD == 0 ? 0.282094791773878143 : LD == 1 ? 0.488602511902919921 : 1.0);
+
+  // pair emptiness: libcint returns early when a pair has no primitive below
+  // the cutoff (src/cint2e.c:96-109)
+  int live_ij = 0, live_kl = 0;
+  for (int p = tl; p < nij; p += T) live_ij |= (pij[p].cce0 - ld_ij) < Ecut;
+  for (int p = tl; p < nkl; p += T) live_kl |= (pkl[p].cce0 - ld_kl) < Ecut;
+  for (int off = 1; off < T; off <<= 1) {
+    live_ij |= __shfl_xor(live_ij, off);
+    live_kl |= __shfl_xor(live_kl, off);
+  }
+  const int live = live_ij && live_kl;
+
+  double acc[NE * NF];
+  sfor<NE * NF>([&](auto I) { acc[decltype(I)::value] = 0.0; });
+  int any = 0;
+
+  if (live) {
+    const int np = nij * nkl;
+    int kl = tl / nij;
+    int ij = tl - kl * nij;
+    for (int p = tl; p < np; p += T) {
+      const PairPrim Q = pkl[kl];
+      const PairPrim P = pij[ij];
+      // advance (ij fastest) before any 'continue'
+      ij += T;
+      while (ij >= nij) {
+        ij -= nij;
+        ++kl;
+      }
+      const double cce_kl = Q.cce0 - ld_kl;
+      if (cce_kl > Ecut) continue;
+      const double cce_ij = P.cce0 - ld_ij;
+      if (cce_ij > Ecut - cce_kl) continue;
+      any = 1;
+      if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
+
+      const double aij = P.aij, akl = Q.aij;
+      const double a1 = aij * akl;
+      const double a0 = a1 / (aij + akl);
+      const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
+      const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
+      const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * Q.kab;
+      double u[NR], w[NR];
+      rys_roots<NR>(a0 * rr, u, w);
+      const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
+      const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
+      sfor<NR>([&](auto RI) {
+        constexpr int r = decltype(RI)::value;
+        const double u2 = a0 * u[r];
+        const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+        const double b00 = u2 * tmp4;
+        const double b10 = b00 + tmp4 * akl;
+        const double b01 = b00 + tmp4 * aij;
+        const double tmp2 = 2.0 * b00 * akl;
+        const double tmp3 = 2.0 * b00 * aij;
+        double Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
+        vrr2d<E, F>(Vx, 1.0, pax - tmp2 * xpq, qcx + tmp3 * xpq, b10, b01, b00);
+        vrr2d<E, F>(Vy, 1.0, pay - tmp2 * ypq, qcy + tmp3 * ypq, b10, b01, b00);
+        vrr2d<E, F>(Vz, w[r] * fac1, paz - tmp2 * zpq, qcz + tmp3 * zpq, b10, b01, b00);
+        sfor<NE>([&](auto II) {
+          constexpr int i = decltype(II)::value;
+          constexpr int ex = cpx(LA, i), ey = cpy(LA, i), ez = cpz(LA, i);
+          sfor<NF>([&](auto JJ) {
+            constexpr int j = decltype(JJ)::value;
+            constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
+            acc[i * NF + j] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[i * NF + j]);
+          });
+        });
+      });
+    }
+  }
+
+  // team sum
+  for (int off = 1; off < T; off <<= 1) {
+    sfor<NE * NF>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      acc[i] += __shfl_xor(acc[i], off);
+    });
+    any |= __shfl_xor(any, off);
+  }
+  any = any && live;
+  if (tl == 0 && A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
+
+  // ---- epilogue --------------------------------------------------------
+  const int perm = A.perm;
+  int64_t so[4];
+  {
+    const int d0 = sa.dsph, d1 = sb.dsph, d2 = sc.dsph, d3 = sd.dsph;
+    int dorig[4];
+    // canonical axis a holds original axis (perm >> 2a) & 3
+    dorig[perm & 3] = d0;
+    dorig[(perm >> 2) & 3] = d1;
+    dorig[(perm >> 4) & 3] = d2;
+    dorig[(perm >> 6) & 3] = d3;
+    so[0] = 1;
+    so[1] = dorig[0];
+    so[2] = so[1] * dorig[1];
+    so[3] = so[2] * dorig[2];
+  }
+  const int64_t st0 = so[perm & 3], st1 = so[(perm >> 2) & 3];
+  const int64_t st2 = so[(perm >> 4) & 3], st3 = so[(perm >> 6) & 3];
+  double* __restrict__ out = A.out + tk.out;
+
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  if (!any) {
+    sfor<NAo * NBo * NCo * NDo>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr int ao = k / (NBo * NCo * NDo), bo = (k / (NCo * NDo)) % NBo;
+      constexpr int co = (k / NDo) % NCo, dd = k % NDo;
+      if ((k & (T - 1)) == tl) out[ao * st0 + bo * st1 + co * st2 + dd * st3] = 0.0;
+    });
+    return;
+  }
+  // powers of A-B and C-D
+  double abp[3][LB + 1], cdp[3][LD + 1];
+  {
+    const double ab[3] = {Ax - sb.r[0], Ay - sb.r[1], Az - sb.r[2]};
+    const double cd[3] = {Cx - sd.r[0], Cy - sd.r[1], Cz - sd.r[2]};
+    sfor<3>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      abp[d][0] = 1.0;
+      cdp[d][0] = 1.0;
+      sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
+      sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
+    });
+  }
+  // bra: (a,b| = sum_t C(b,t) (A-B)^(b-t) [a+t|, then spherical over a and b
+  double Ys[NAo * NBo * NF];
+  sfor<NF>([&](auto JJ) {
+    constexpr int j = decltype(JJ)::value;
+    double Yc[NA * NB];
+    sfor<NA * NB>([&](auto AB) {
+      constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+      constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
+      constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
+      double s = 0.0;
+      sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
+        constexpr int tt = decltype(TT)::value;
+        constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
+        constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
+        constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
+        const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
+        s = fma(cb * pw, acc[e * NF + j], s);
+      });
+      Yc[a * NB + b] = s;
+    });
+    sfor<NAo * NBo>([&](auto OB) {
+      constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
+      double s = 0.0;
+      sfor<NA * NB>([&](auto AB) {
+        constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+        constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+        if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
+      });
+      Ys[(ao * NBo + bo) * NF + j] = s;
+    });
+  });
+  // ket, row by row, then the store (lane tl writes every T-th element)
+  sfor<NAo * NBo>([&](auto RW) {
+    constexpr int row = decltype(RW)::value;
+    constexpr int ao = row / NBo, bo = row % NBo;
+    double Zc[NC * ND];
+    sfor<NC * ND>([&](auto CD) {
+      constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+      constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
+      constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
+      double s = 0.0;
+      sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
+        constexpr int tt = decltype(TT)::value;
+        constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
+        constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
+        constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
+        const double pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+        s = fma(cb * pw, Ys[row * NF + f], s);
+      });
+      Zc[c * ND + d] = s;
+    });
+    sfor<NCo * NDo>([&](auto OD) {
+      constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+      constexpr int k = (row * NCo + co) * NDo + dd;
+      if ((k & (T - 1)) == tl) {
+        double s = 0.0;
+        sfor<NC * ND>([&](auto CD) {
+          constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+          constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+          if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
+        });
+        out[ao * st0 + bo * st1 + co * st2 + dd * st3] = s;
+      }
+    });
+  });
+}
+
+// Classes with a fast kernel (canonical: LA>=LB, LC>=LD, (LA,LB)>=(LC,LD),
+// [e0|f0] block of at most 160 doubles), spherical output of plain 4c ERIs.
+// The list is split in groups, one translation unit each (fast_<g>.hip).
+#define MCINT_FAST_GROUP0(X) X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(2, 0, 0, 0) X(3, 0, 0, 0)
+#define MCINT_FAST_GROUP1(X) X(1, 1, 1, 0) X(2, 0, 1, 0) X(2, 1, 0, 0) X(2, 2, 0, 0) X(3, 1, 0, 0)
+#define MCINT_FAST_GROUP2(X) X(1, 1, 1, 1) X(3, 0, 1, 0) X(3, 2, 0, 0)
+#define MCINT_FAST_GROUP3(X) X(2, 0, 1, 1) X(2, 1, 1, 0) X(3, 3, 0, 0)
+#define MCINT_FAST_GROUP4(X) X(2, 1, 1, 1) X(2, 0, 2, 0)
+#define MCINT_FAST_GROUP5(X) X(2, 1, 2, 0) X(3, 0, 1, 1)
+#define MCINT_FAST_GROUP6(X) X(2, 2, 1, 0) X(3, 0, 2, 0) X(3, 1, 1, 0)
+#define MCINT_FAST_GROUP7(X) X(3, 0, 2, 1) X(3, 0, 3, 0)
+#define MCINT_FAST_GROUP8(X) X(3, 1, 2, 0) X(3, 2, 1, 0)
+#define MCINT_FAST_CLASSES(X)                                                         \
+  MCINT_FAST_GROUP0(X) MCINT_FAST_GROUP1(X) MCINT_FAST_GROUP2(X) MCINT_FAST_GROUP3(X) \
+  MCINT_FAST_GROUP4(X) MCINT_FAST_GROUP5(X) MCINT_FAST_GROUP6(X) MCINT_FAST_GROUP7(X) \
+  MCINT_FAST_GROUP8(X)
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+
+// explicit instantiation of one class (used by fast_<g>.hip)
+#define MCINT_FAST_DEFINE(a, b, c, d)                                                      \
+  template <>                                                                              \
+  hipError_t launch_fast_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,        \
+                                           hipStream_t stream) {                           \
+    const int64_t n = A.t1 - A.t0;                                                         \
+    const int64_t per = 256 / A.team;                                                      \
+    const dim3 grid((unsigned)((n + per - 1) / per));                                      \
+    hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d>), grid, dim3(256), 0, stream, B, A);   \
+    return hipGetLastError();                                                              \
+  }
+
+#define MCINT_FAST_DECLARE(a, b, c, d)                                                     \
+  template <>                                                                              \
+  hipError_t launch_fast_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,        \
+                                           hipStream_t stream);
+
+// dispatch functions (expanded once, in eri.hip)
+#define MC_FAST_HAS(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return true;
+#define MC_FAST_LAUNCH(a, b, c, d) \
+  if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_fast_class<a, b, c, d>(B, A, stream);
+#define MCINT_FAST_DISPATCH                                                                \
+  MCINT_FAST_CLASSES(MCINT_FAST_DECLARE)                                                   \
+  bool fast_kernel_available(int kind, int sph, int la, int lb, int lc, int ld) {          \
+    if (kind != KIND_2E || !sph) return false;                                             \
+    MCINT_FAST_CLASSES(MC_FAST_HAS)                                                        \
+    return false;                                                                          \
+  }                                                                                        \
+  int fast_team_lanes_max() { return 64; }                                                 \
+  hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream) {    \
+    if (A.t1 <= A.t0) return hipSuccess;                                                   \
+    if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
+    MCINT_FAST_CLASSES(MC_FAST_LAUNCH)                                                     \
+    return hipErrorNotSupported;                                                           \
+  }
 
 }  // namespace mcint

@@ -77,6 +77,9 @@ hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const i
 hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, double* w,
                             hipStream_t stream);
 
+hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned long long* counter,
+                              hipStream_t stream);
+
 hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
 // Register-resident kernels for the classes listed in eri_fast.h.

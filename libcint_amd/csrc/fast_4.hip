@@ -1,0 +1,7 @@
+// Fast-kernel instantiations, group 4 (see eri_fast.h).
+#include "device_common.h"
+#include "eri_fast.h"
+
+namespace mcint {
+MCINT_FAST_GROUP4(MCINT_FAST_DEFINE)
+}  // namespace mcint

@@ -1,0 +1,7 @@
+// Fast-kernel instantiations, group 7 (see eri_fast.h).
+#include "device_common.h"
+#include "eri_fast.h"
+
+namespace mcint {
+MCINT_FAST_GROUP7(MCINT_FAST_DEFINE)
+}  // namespace mcint
