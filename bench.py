@@ -157,7 +157,7 @@ def main():
     top = sorted(range(len(buckets)), key=lambda i: -btime[i])[:8]
     for i in top:
         b = buckets[i]
-        log(f"bucket {i}: cls={b['cls']} team={b['team']} fast={b['fast']} tasks={b['tasks']} "
+        log(f"bucket {i}: cls={b['cls']} team={b['team']} {b['path']} tasks={b['tasks']} "
             f"prims={nprim[i]} t={btime[i] * 1e3:.2f} ms  {bflops[i] / btime[i] / 1e12:.2f} TFLOP/s")
 
     result = {
@@ -195,7 +195,7 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
             "kernel": f"class {buckets[dom]['cls']} team {buckets[dom]['team']} "
-                      f"({'fast' if buckets[dom]['fast'] else 'generic'}), "
+                      f"({buckets[dom]['path']}), "
                       f"{buckets[dom]['tasks']} quartets, {int(nprim[dom])} primitive quartets",
             "launch_ms": dom_t * 1e3,
             "flops_per_launch": dom_flops,

@@ -175,7 +175,8 @@ void mcint_plan_destroy(mcint_plan *p);
 int mcint_plan_stats(const mcint_plan *p, int64_t *n);
 
 /* Buckets of a plan (one kernel launch each).  info receives: la, lb, lc,
- * ld (canonical class), perm, team lanes, fast(1)/generic(0), task count. */
+ * ld (canonical class), perm, team lanes, kernel path (0 generic, 1 single
+ * register kernel, 2 chunked), task count. */
 int mcint_plan_nbuckets(const mcint_plan *p);
 int mcint_plan_bucket_info(const mcint_plan *p, int bucket, int64_t *info);
 int mcint_plan_execute_bucket(mcint_plan *p, int bucket, double *out_dev, FINT *not0_dev,

@@ -238,14 +238,14 @@ class Plan:
                     prim_quartets=n[4], out_size=n[5])
 
     def buckets(self) -> list:
-        """Per bucket: dict(cls=(la,lb,lc,ld), perm, team, fast, tasks)."""
+        """Per bucket: dict(cls=(la,lb,lc,ld), perm, team, path, tasks)."""
         L = lib()
         out = []
         info = (ctypes.c_int64 * 8)()
         for i in range(L.mcint_plan_nbuckets(self.handle)):
             _check(L.mcint_plan_bucket_info(self.handle, i, info), "mcint_plan_bucket_info")
-            out.append(dict(cls=tuple(info[:4]), perm=info[4], team=info[5], fast=bool(info[6]),
-                            tasks=info[7]))
+            out.append(dict(cls=tuple(info[:4]), perm=info[4], team=info[5],
+                            path=("generic", "fast", "chunked")[info[6]], tasks=info[7]))
         return out
 
     def count_primitives(self) -> np.ndarray:

@@ -182,13 +182,14 @@ namespace {
 
 struct Bucket {
   int la, lb, lc, ld, perm, team;
-  bool fast;
+  int mode;  // 0 generic, 1 fast (single chunk), 2 chunked
   int64_t t0, t1;
   int64_t sstride;  // generic scratch doubles per task
 };
 
-// Largest number of generic tasks launched at once (bounds the scratch).
-constexpr int64_t GEN_SCRATCH_DOUBLES = 32ll << 20;
+// Scratch bound of the generic and chunked paths (doubles); buckets larger
+// than the scratch are launched in slices.
+constexpr int64_t GEN_SCRATCH_DOUBLES = 64ll << 20;
 
 int pick_team(int64_t nprim_quartets) {
   int t = 1;
@@ -210,6 +211,8 @@ struct mcint_plan {
   PairPrim* pp_d = nullptr;
   double* scratch_d = nullptr;
   int64_t scratch_size = 0;
+  int* flags_d = nullptr;
+  int64_t flags_size = 0;
   int64_t stats[6] = {0, 0, 0, 0, 0, 0};
   // graph cache
   hipGraph_t graph = nullptr;
@@ -229,6 +232,7 @@ static void plan_free(mcint_plan* p) {
   dfree(p->pair_ld_d);
   dfree(p->pp_d);
   dfree(p->scratch_d);
+  dfree(p->flags_d);
   delete p;
 }
 
@@ -334,13 +338,13 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
     const int la = l[L[0]], lb = l[L[1]], lc = l[L[2]], ld = kind == KIND_3C2E ? 0 : l[L[3]];
     const int64_t nprimq = (int64_t)np[0] * np[1] * np[2] * (naxes == 4 ? np[3] : 1);
     nprimq_total += nprimq;
-    const bool fast = fast_kernel_available(kind, sph, la, lb, lc, ld);
-    const int team = fast ? pick_team(nprimq) : 0;
+    const int mode = fast_kernel_mode(kind, sph, la, lb, lc, ld);
+    const int team = mode ? pick_team(nprimq) : 0;
     int tl = 0;
     while ((1 << tl) < team) ++tl;
     const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |
                        ((uint32_t)ld << 9) | ((uint32_t)perm << 12) | ((uint32_t)tl << 20) |
-                       ((uint32_t)fast << 24);
+                       ((uint32_t)mode << 24);
     // virtual expansion over contraction columns of the original axes
     int cidx[4] = {0, 0, 0, 0};
     for (;;) {
@@ -409,7 +413,7 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
       sorted_q[d] = task_q[t];
     }
   }
-  int64_t max_gen = 0;
+  int64_t max_gen = 0, max_flags = 0;
   for (size_t i = 0; i < ukeys.size(); ++i) {
     const uint32_t k = ukeys[i];
     Bucket bk;
@@ -419,11 +423,18 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
     bk.ld = (k >> 9) & 7;
     bk.perm = (k >> 12) & 255;
     bk.team = 1 << ((k >> 20) & 15);
-    bk.fast = (k >> 24) & 1;
+    bk.mode = (k >> 24) & 3;
     bk.t0 = cnt[i];
     bk.t1 = cnt[i + 1];
     bk.sstride = 0;
-    if (!bk.fast) {
+    if (bk.mode == 2) {
+      const int64_t nef = chunked_block_doubles(bk.la, bk.lb, bk.lc, bk.ld);
+      bk.sstride = nef;
+      const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(bk.t1 - bk.t0, GEN_SCRATCH_DOUBLES / nef));
+      max_gen = std::max(max_gen, chunk * nef);
+      max_flags = std::max(max_flags, chunk);
+      p->stats[2] += bk.t1 - bk.t0;
+    } else if (bk.mode == 0) {
       const GenLayout G = gen_layout(kind, sph, bk.la, bk.lb, bk.lc, bk.ld);
       if (G.nE > 220 || G.nF > 165) {
         plan_free(p);
@@ -470,7 +481,9 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
   if (e == hipSuccess) e = dalloc(&p->pp_d, (size_t)std::max<int64_t>(npp, 1));
   if (e == hipSuccess) e = dalloc(&pairs_d, pairs.size());
   if (e == hipSuccess && max_gen > 0) e = dalloc(&p->scratch_d, (size_t)max_gen);
+  if (e == hipSuccess && max_flags > 0) e = dalloc(&p->flags_d, (size_t)max_flags);
   p->scratch_size = max_gen;
+  p->flags_size = max_flags;
   if (e == hipSuccess && !sorted.empty())
     e = hipMemcpy(p->tasks_d, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice);
   if (e == hipSuccess && !sorted_q.empty())
@@ -522,6 +535,7 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.not0 = not0;
   A.task_id = p->task_id_d;
   A.scratch = p->scratch_d;
+  A.flags = p->flags_d;
   A.scratch_stride = bk.sstride;
   A.la = bk.la;
   A.lb = bk.lb;
@@ -540,8 +554,15 @@ static int bucket_launch(mcint_plan* p, const Bucket& bk, double* out, FINT* not
                          hipStream_t stream) {
   const BasisView B = plan_view(p);
   BucketArgs A = bucket_args(p, bk, out, not0);
-  if (bk.fast) {
+  if (bk.mode == 1) {
     HIPCHK(launch_fast(B, A, stream));
+  } else if (bk.mode == 2) {
+    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
+    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+      A.t0 = t;
+      A.t1 = std::min(bk.t1, t + chunk);
+      HIPCHK(launch_chunked(B, A, stream));
+    }
   } else {
     const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
     for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
@@ -569,7 +590,7 @@ extern "C" int mcint_plan_bucket_info(const mcint_plan* p, int i, int64_t* info)
     return MCINT_ERR_ARG;
   }
   const Bucket& b = p->buckets[i];
-  const int64_t v[8] = {b.la, b.lb, b.lc, b.ld, b.perm, b.team, b.fast ? 1 : 0, b.t1 - b.t0};
+  const int64_t v[8] = {b.la, b.lb, b.lc, b.ld, b.perm, b.team, b.mode, b.t1 - b.t0};
   for (int k = 0; k < 8; ++k) info[k] = v[k];
   return MCINT_OK;
 }

@@ -125,17 +125,15 @@ __device__ __forceinline__ void vrr2d(double (&V)[E + 1][F + 1], double v0, doub
   });
 }
 
-template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
+// Primitive loop of one team: accumulates the [e0|f0] components with flat
+// index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
+// is the team's pair-emptiness verdict, 'any' whether this lane saw a
+// primitive quartet that passed libcint's two screening tests.
+template <int LA, int LB, int LC, int LD, int K0, int K1>
+__device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
+                                           double (&acc)[K1 - K0], int& any, int& live) {
   using C = FastClass<LA, LB, LC, LD>;
-  constexpr int E = C::E, F = C::F, NR = C::NR, NE = C::NE, NF = C::NF;
-  const int T = A.team;
-  const int tid = threadIdx.x;
-  const int tl = tid & (T - 1);
-  const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
-  if (t >= A.t1) return;  // whole teams leave together
-
-  const Task tk = A.tasks[t];
+  constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
   const VShell& sa = Bv.vsh[tk.sh[0]];
   const VShell& sb = Bv.vsh[tk.sh[1]];
   const VShell& sc = Bv.vsh[tk.sh[2]];
@@ -167,67 +165,104 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
     live_ij |= __shfl_xor(live_ij, off);
     live_kl |= __shfl_xor(live_kl, off);
   }
-  const int live = live_ij && live_kl;
+  live = live_ij && live_kl;
 
-  double acc[NE * NF];
-  sfor<NE * NF>([&](auto I) { acc[decltype(I)::value] = 0.0; });
-  int any = 0;
-
-  if (live) {
-    const int np = nij * nkl;
-    int kl = tl / nij;
-    int ij = tl - kl * nij;
-    for (int p = tl; p < np; p += T) {
-      const PairPrim Q = pkl[kl];
-      const PairPrim P = pij[ij];
-      // advance (ij fastest) before any 'continue'
-      ij += T;
-      while (ij >= nij) {
-        ij -= nij;
-        ++kl;
-      }
-      const double cce_kl = Q.cce0 - ld_kl;
-      if (cce_kl > Ecut) continue;
-      const double cce_ij = P.cce0 - ld_ij;
-      if (cce_ij > Ecut - cce_kl) continue;
-      any = 1;
-      if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
-
-      const double aij = P.aij, akl = Q.aij;
-      const double a1 = aij * akl;
-      const double a0 = a1 / (aij + akl);
-      const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
-      const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
-      const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * Q.kab;
-      double u[NR], w[NR];
-      rys_roots<NR>(a0 * rr, u, w);
-      const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
-      const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
-      sfor<NR>([&](auto RI) {
-        constexpr int r = decltype(RI)::value;
-        const double u2 = a0 * u[r];
-        const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
-        const double b00 = u2 * tmp4;
-        const double b10 = b00 + tmp4 * akl;
-        const double b01 = b00 + tmp4 * aij;
-        const double tmp2 = 2.0 * b00 * akl;
-        const double tmp3 = 2.0 * b00 * aij;
-        double Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
-        vrr2d<E, F>(Vx, 1.0, pax - tmp2 * xpq, qcx + tmp3 * xpq, b10, b01, b00);
-        vrr2d<E, F>(Vy, 1.0, pay - tmp2 * ypq, qcy + tmp3 * ypq, b10, b01, b00);
-        vrr2d<E, F>(Vz, w[r] * fac1, paz - tmp2 * zpq, qcz + tmp3 * zpq, b10, b01, b00);
-        sfor<NE>([&](auto II) {
-          constexpr int i = decltype(II)::value;
-          constexpr int ex = cpx(LA, i), ey = cpy(LA, i), ez = cpz(LA, i);
-          sfor<NF>([&](auto JJ) {
-            constexpr int j = decltype(JJ)::value;
-            constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
-            acc[i * NF + j] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[i * NF + j]);
-          });
-        });
-      });
+  sfor<K1 - K0>([&](auto I) { acc[decltype(I)::value] = 0.0; });
+  any = 0;
+  if (!live) return;
+  const int np = nij * nkl;
+  int kl = tl / nij;
+  int ij = tl - kl * nij;
+  for (int p = tl; p < np; p += T) {
+    const PairPrim Q = pkl[kl];
+    const PairPrim P = pij[ij];
+    // advance (ij fastest) before any 'continue'
+    ij += T;
+    while (ij >= nij) {
+      ij -= nij;
+      ++kl;
     }
+    const double cce_kl = Q.cce0 - ld_kl;
+    if (cce_kl > Ecut) continue;
+    const double cce_ij = P.cce0 - ld_ij;
+    if (cce_ij > Ecut - cce_kl) continue;
+    any = 1;
+    if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
+
+    const double aij = P.aij, akl = Q.aij;
+    const double a1 = aij * akl;
+    const double a0 = a1 / (aij + akl);
+    const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
+    const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
+    const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * Q.kab;
+    double u[NR], w[NR];
+    rys_roots<NR>(a0 * rr, u, w);
+    const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
+    const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
+    sfor<NR>([&](auto RI) {
+      constexpr int r = decltype(RI)::value;
+      const double u2 = a0 * u[r];
+      const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+      const double b00 = u2 * tmp4;
+      const double b10 = b00 + tmp4 * akl;
+      const double b01 = b00 + tmp4 * aij;
+      const double tmp2 = 2.0 * b00 * akl;
+      const double tmp3 = 2.0 * b00 * aij;
+      double Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
+      vrr2d<E, F>(Vx, 1.0, pax - tmp2 * xpq, qcx + tmp3 * xpq, b10, b01, b00);
+      vrr2d<E, F>(Vy, 1.0, pay - tmp2 * ypq, qcy + tmp3 * ypq, b10, b01, b00);
+      vrr2d<E, F>(Vz, w[r] * fac1, paz - tmp2 * zpq, qcz + tmp3 * zpq, b10, b01, b00);
+      sfor<K1 - K0>([&](auto KK) {
+        constexpr int k = K0 + decltype(KK)::value;
+        constexpr int i = k / NF, j = k % NF;
+        constexpr int ex = cpx(LA, i), ey = cpy(LA, i), ez = cpz(LA, i);
+        constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
+        acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
+      });
+    });
   }
+}
+
+// Output strides of the canonical axes: axis a holds the original axis
+// (perm >> 2a) & 3 of the packed Fortran block.
+__device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& tk, int perm,
+                                              int64_t& st0, int64_t& st1, int64_t& st2,
+                                              int64_t& st3) {
+  int dorig[4];
+  dorig[perm & 3] = Bv.vsh[tk.sh[0]].dsph;
+  dorig[(perm >> 2) & 3] = Bv.vsh[tk.sh[1]].dsph;
+  dorig[(perm >> 4) & 3] = Bv.vsh[tk.sh[2]].dsph;
+  dorig[(perm >> 6) & 3] = Bv.vsh[tk.sh[3]].dsph;
+  int64_t so[4];
+  so[0] = 1;
+  so[1] = dorig[0];
+  so[2] = so[1] * dorig[1];
+  so[3] = so[2] * dorig[2];
+  st0 = so[perm & 3];
+  st1 = so[(perm >> 2) & 3];
+  st2 = so[(perm >> 4) & 3];
+  st3 = so[(perm >> 6) & 3];
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD>;
+  constexpr int NE = C::NE, NF = C::NF;
+  const int T = A.team;
+  const int tid = threadIdx.x;
+  const int tl = tid & (T - 1);
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
+  if (t >= A.t1) return;  // whole teams leave together
+  const Task tk = A.tasks[t];
+  const VShell& sa = Bv.vsh[tk.sh[0]];
+  const VShell& sb = Bv.vsh[tk.sh[1]];
+  const VShell& sc = Bv.vsh[tk.sh[2]];
+  const VShell& sd = Bv.vsh[tk.sh[3]];
+  const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
+  const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
+  double acc[NE * NF];
+  int any, live;
+  accumulate<LA, LB, LC, LD, 0, NE * NF>(Bv, tk, T, tl, acc, any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -241,23 +276,8 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   if (tl == 0 && A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
 
   // ---- epilogue --------------------------------------------------------
-  const int perm = A.perm;
-  int64_t so[4];
-  {
-    const int d0 = sa.dsph, d1 = sb.dsph, d2 = sc.dsph, d3 = sd.dsph;
-    int dorig[4];
-    // canonical axis a holds original axis (perm >> 2a) & 3
-    dorig[perm & 3] = d0;
-    dorig[(perm >> 2) & 3] = d1;
-    dorig[(perm >> 4) & 3] = d2;
-    dorig[(perm >> 6) & 3] = d3;
-    so[0] = 1;
-    so[1] = dorig[0];
-    so[2] = so[1] * dorig[1];
-    so[3] = so[2] * dorig[2];
-  }
-  const int64_t st0 = so[perm & 3], st1 = so[(perm >> 2) & 3];
-  const int64_t st2 = so[(perm >> 4) & 3], st3 = so[(perm >> 6) & 3];
+  int64_t st0, st1, st2, st3;
+  canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
   double* __restrict__ out = A.out + tk.out;
 
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
@@ -351,9 +371,219 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   });
 }
 
-// Classes with a fast kernel (canonical: LA>=LB, LC>=LD, (LA,LB)>=(LC,LD),
-// [e0|f0] block of at most 160 doubles), spherical output of plain 4c ERIs.
-// The list is split in groups, one translation unit each (fast_<g>.hip).
+// ---------------------------------------------------------------------
+// Chunked path for classes whose [e0|f0] block exceeds the register budget:
+// chunk kernels (one per compile-time component range) accumulate their
+// range in registers and store it to a per-task scratch block; the
+// epilogue kernel then runs the transfers and the spherical transform
+// cooperatively in LDS.
+// ---------------------------------------------------------------------
+constexpr int CHUNK_TARGET = 96;
+
+template <int LA, int LB, int LC, int LD>
+struct ChunkCfg {
+  using C = FastClass<LA, LB, LC, LD>;
+  static constexpr int NEF = C::NE * C::NF;
+  static constexpr int NCH = (NEF + CHUNK_TARGET - 1) / CHUNK_TARGET;
+  static constexpr int W = (NEF + NCH - 1) / NCH;
+  static constexpr int k0(int c) { return c * W; }
+  static constexpr int k1(int c) { return (c + 1) * W < NEF ? (c + 1) * W : NEF; }
+};
+
+template <int LA, int LB, int LC, int LD, int CH>
+__global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs A) {
+  using G = ChunkCfg<LA, LB, LC, LD>;
+  constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
+  const int T = A.team;
+  const int tid = threadIdx.x;
+  const int tl = tid & (T - 1);
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
+  if (t >= A.t1) return;
+  const Task tk = A.tasks[t];
+  double acc[K1 - K0];
+  int any, live;
+  accumulate<LA, LB, LC, LD, K0, K1>(Bv, tk, T, tl, acc, any, live);
+  for (int off = 1; off < T; off <<= 1) {
+    sfor<K1 - K0>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      acc[i] += __shfl_xor(acc[i], off);
+    });
+    any |= __shfl_xor(any, off);
+  }
+  const int64_t slot = t - A.t0;
+  double* __restrict__ dst = A.scratch + slot * G::NEF;
+  sfor<K1 - K0>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if ((i & (T - 1)) == tl) dst[K0 + i] = acc[i];
+  });
+  if constexpr (CH == 0) {
+    if (tl == 0) {
+      A.flags[slot] = any && live;
+      if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any && live;
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+struct EpiCfg {
+  using C = FastClass<LA, LB, LC, LD>;
+  static constexpr int NEF = C::NE * C::NF;
+  static constexpr int PER = NEF + C::NAo * C::NBo * C::NF;  // doubles of LDS per task
+  static constexpr int WORK = C::NF > C::NAo * C::NBo ? C::NF : C::NAo * C::NBo;
+  static constexpr int TE = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
+  static constexpr int TPB0 = 256 / TE;
+  static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
+  static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
+  static constexpr int THREADS = TPB * TE;
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD>;
+  using P = EpiCfg<LA, LB, LC, LD>;
+  constexpr int NE = C::NE, NF = C::NF, NEF = P::NEF, TE = P::TE;
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  __shared__ double sm[P::TPB * P::PER];
+  const int team = threadIdx.x / TE;
+  const int tl = threadIdx.x % TE;
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * P::TPB + team;
+  const bool valid = t < A.t1;
+  double* X = sm + team * P::PER;
+  double* Ys = X + NEF;
+  int any = 0;
+  Task tk;
+  if (valid) {
+    tk = A.tasks[t];
+    const int64_t slot = t - A.t0;
+    any = A.flags[slot];
+    const double* src = A.scratch + slot * NEF;
+    if (any)
+      for (int k = tl; k < NEF; k += TE) X[k] = src[k];
+  }
+  __syncthreads();
+  if (valid && any) {
+    const VShell& sa = Bv.vsh[tk.sh[0]];
+    const VShell& sb = Bv.vsh[tk.sh[1]];
+    const double ab[3] = {sa.r[0] - sb.r[0], sa.r[1] - sb.r[1], sa.r[2] - sb.r[2]};
+    double abp[3][LB + 1];
+    sfor<3>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      abp[d][0] = 1.0;
+      sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
+    });
+    for (int j = tl; j < NF; j += TE) {
+      double Yc[NA * NB];
+      sfor<NA * NB>([&](auto AB) {
+        constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+        constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
+        constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
+        double s = 0.0;
+        sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
+          constexpr int tt = decltype(TT)::value;
+          constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
+          constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
+          constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
+          const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
+          s = fma(cb * pw, X[e * NF + j], s);
+        });
+        Yc[a * NB + b] = s;
+      });
+      sfor<NAo * NBo>([&](auto OB) {
+        constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
+        double s = 0.0;
+        sfor<NA * NB>([&](auto AB) {
+          constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+          constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+          if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
+        });
+        Ys[(ao * NBo + bo) * NF + j] = s;
+      });
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  int64_t st0, st1, st2, st3;
+  canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
+  double* __restrict__ out = A.out + tk.out;
+  if (!any) {
+    for (int k = tl; k < NAo * NBo * NCo * NDo; k += TE) {
+      const int ao = k % NAo, bo = (k / NAo) % NBo, co = (k / (NAo * NBo)) % NCo;
+      const int dd = k / (NAo * NBo * NCo);
+      out[ao * st0 + bo * st1 + co * st2 + dd * st3] = 0.0;
+    }
+    return;
+  }
+  const VShell& sc = Bv.vsh[tk.sh[2]];
+  const VShell& sd = Bv.vsh[tk.sh[3]];
+  const double cd[3] = {sc.r[0] - sd.r[0], sc.r[1] - sd.r[1], sc.r[2] - sd.r[2]};
+  double cdp[3][LD + 1];
+  sfor<3>([&](auto DI) {
+    constexpr int d = decltype(DI)::value;
+    cdp[d][0] = 1.0;
+    sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
+  });
+  for (int row = tl; row < NAo * NBo; row += TE) {
+    const int ao = row / NBo, bo = row % NBo;
+    const double* yr = Ys + row * NF;
+    double Zc[NC * ND];
+    sfor<NC * ND>([&](auto CD) {
+      constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+      constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
+      constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
+      double s = 0.0;
+      sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
+        constexpr int tt = decltype(TT)::value;
+        constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
+        constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
+        constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
+        const double pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+        s = fma(cb * pw, yr[f], s);
+      });
+      Zc[c * ND + d] = s;
+    });
+    double* orow = out + ao * st0 + bo * st1;
+    sfor<NCo * NDo>([&](auto OD) {
+      constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+      double s = 0.0;
+      sfor<NC * ND>([&](auto CD) {
+        constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+        constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+        if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
+      });
+      orow[co * st2 + dd * st3] = s;
+    });
+  }
+}
+
+template <int LA, int LB, int LC, int LD, int CH>
+hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t stream, dim3 grid) {
+  hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH>), grid, dim3(256), 0, stream, B, A);
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD, int... CHs>
+hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t stream,
+                         std::integer_sequence<int, CHs...>) {
+  const int64_t n = A.t1 - A.t0;
+  const int64_t per = 256 / A.team;
+  const dim3 grid((unsigned)((n + per - 1) / per));
+  hipError_t e = hipSuccess;
+  ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs>(B, A, stream, grid) : e)), ...);
+  if (e != hipSuccess) return e;
+  using P = EpiCfg<LA, LB, LC, LD>;
+  const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+  hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+
+// Classes with a single-chunk fast kernel (canonical: LA>=LB, LC>=LD,
+// (LA,LB)>=(LC,LD), [e0|f0] block of at most 160 doubles), spherical output
+// of plain 4c ERIs.  The list is split in groups, one translation unit each
+// (fast_<g>.hip).
 #define MCINT_FAST_GROUP0(X) X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(2, 0, 0, 0) X(3, 0, 0, 0)
 #define MCINT_FAST_GROUP1(X) X(1, 1, 1, 0) X(2, 0, 1, 0) X(2, 1, 0, 0) X(2, 2, 0, 0) X(3, 1, 0, 0)
 #define MCINT_FAST_GROUP2(X) X(1, 1, 1, 1) X(3, 0, 1, 0) X(3, 2, 0, 0)
@@ -367,6 +597,24 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   MCINT_FAST_GROUP0(X) MCINT_FAST_GROUP1(X) MCINT_FAST_GROUP2(X) MCINT_FAST_GROUP3(X) \
   MCINT_FAST_GROUP4(X) MCINT_FAST_GROUP5(X) MCINT_FAST_GROUP6(X) MCINT_FAST_GROUP7(X) \
   MCINT_FAST_GROUP8(X)
+
+// Classes on the chunked path ([e0|f0] block of at most 800 doubles), one
+// translation unit per group (chunk_<g>.hip).
+#define MCINT_CHUNK_GROUP0(X) X(3, 1, 1, 1) X(3, 1, 3, 0)
+#define MCINT_CHUNK_GROUP1(X) X(2, 1, 2, 1) X(2, 2, 2, 0)
+#define MCINT_CHUNK_GROUP2(X) X(2, 2, 1, 1) X(3, 3, 1, 0)
+#define MCINT_CHUNK_GROUP3(X) X(3, 2, 1, 1) X(3, 2, 2, 0)
+#define MCINT_CHUNK_GROUP4(X) X(3, 1, 2, 1) X(3, 0, 2, 2)
+#define MCINT_CHUNK_GROUP5(X) X(2, 2, 2, 1) X(3, 3, 2, 0)
+#define MCINT_CHUNK_GROUP6(X) X(3, 1, 3, 1) X(3, 2, 3, 0)
+#define MCINT_CHUNK_GROUP7(X) X(3, 3, 1, 1)
+#define MCINT_CHUNK_GROUP8(X) X(3, 2, 2, 1)
+#define MCINT_CHUNK_GROUP9(X) X(3, 3, 3, 0)
+#define MCINT_CHUNK_GROUP10(X) X(3, 1, 2, 2)
+#define MCINT_CHUNK_CLASSES(X)                                                            \
+  MCINT_CHUNK_GROUP0(X) MCINT_CHUNK_GROUP1(X) MCINT_CHUNK_GROUP2(X) MCINT_CHUNK_GROUP3(X) \
+  MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X) \
+  MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X)
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
@@ -382,29 +630,58 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
     hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d>), grid, dim3(256), 0, stream, B, A);   \
     return hipGetLastError();                                                              \
   }
+#define MCINT_CHUNK_DEFINE(a, b, c, d)                                                     \
+  template <>                                                                              \
+  hipError_t launch_chunked_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,     \
+                                              hipStream_t stream) {                        \
+    return launch_chunks<a, b, c, d>(                                                      \
+        B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d>::NCH>{});       \
+  }
 
 #define MCINT_FAST_DECLARE(a, b, c, d)                                                     \
   template <>                                                                              \
   hipError_t launch_fast_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,        \
                                            hipStream_t stream);
+#define MCINT_CHUNK_DECLARE(a, b, c, d)                                                    \
+  template <>                                                                              \
+  hipError_t launch_chunked_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,     \
+                                              hipStream_t stream);
 
 // dispatch functions (expanded once, in eri.hip)
 #define MC_FAST_HAS(a, b, c, d) \
-  if (la == a && lb == b && lc == c && ld == d) return true;
+  if (la == a && lb == b && lc == c && ld == d) return 1;
+#define MC_CHUNK_HAS(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return 2;
 #define MC_FAST_LAUNCH(a, b, c, d) \
   if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_fast_class<a, b, c, d>(B, A, stream);
+#define MC_CHUNK_LAUNCH(a, b, c, d) \
+  if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_chunked_class<a, b, c, d>(B, A, stream);
+#define MC_CHUNK_NEF(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return ChunkCfg<a, b, c, d>::NEF;
 #define MCINT_FAST_DISPATCH                                                                \
   MCINT_FAST_CLASSES(MCINT_FAST_DECLARE)                                                   \
-  bool fast_kernel_available(int kind, int sph, int la, int lb, int lc, int ld) {          \
-    if (kind != KIND_2E || !sph) return false;                                             \
+  MCINT_CHUNK_CLASSES(MCINT_CHUNK_DECLARE)                                                  \
+  int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld) {                \
+    if (kind != KIND_2E || !sph) return 0;                                                 \
     MCINT_FAST_CLASSES(MC_FAST_HAS)                                                        \
-    return false;                                                                          \
+    MCINT_CHUNK_CLASSES(MC_CHUNK_HAS)                                                      \
+    return 0;                                                                              \
+  }                                                                                        \
+  int chunked_block_doubles(int la, int lb, int lc, int ld) {                              \
+    MCINT_CHUNK_CLASSES(MC_CHUNK_NEF)                                                      \
+    return 0;                                                                              \
   }                                                                                        \
   int fast_team_lanes_max() { return 64; }                                                 \
   hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream) {    \
     if (A.t1 <= A.t0) return hipSuccess;                                                   \
     if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
     MCINT_FAST_CLASSES(MC_FAST_LAUNCH)                                                     \
+    return hipErrorNotSupported;                                                           \
+  }                                                                                        \
+  hipError_t launch_chunked(const BasisView& B, const BucketArgs& A, hipStream_t stream) { \
+    if (A.t1 <= A.t0) return hipSuccess;                                                   \
+    if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
+    MCINT_CHUNK_CLASSES(MC_CHUNK_LAUNCH)                                                   \
     return hipErrorNotSupported;                                                           \
   }
 

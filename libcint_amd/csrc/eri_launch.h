@@ -82,9 +82,13 @@ hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned 
 
 hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
-// Register-resident kernels for the classes listed in eri_fast.h.
-bool fast_kernel_available(int kind, int sph, int la, int lb, int lc, int ld);
+// Register-resident kernels for the classes listed in eri_fast.h:
+// mode 1 = one kernel with an in-register epilogue, 2 = chunk kernels plus
+// an LDS epilogue (scratch of chunked_block_doubles() per task), 0 = none.
+int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld);
+int chunked_block_doubles(int la, int lb, int lc, int ld);
 int fast_team_lanes_max();
 hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+hipError_t launch_chunked(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
 }  // namespace mcint

@@ -74,7 +74,8 @@ struct BucketArgs {
   double* out;           // output base
   int* not0;             // per task status (nullable), indexed by task id
   const int* task_id;    // original list position of each task (for not0), nullable
-  double* scratch;       // generic path: per-task accumulators
+  double* scratch;       // generic / chunked path: per-task accumulators
+  int* flags;            // chunked path: per-task 'any primitive survived'
   int64_t scratch_stride;
   int la, lb, lc, ld;    // canonical angular momenta
   int kind;
