@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--schwarz", type=float, default=1e-12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the quartet list "
@@ -96,6 +98,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     m = make_molecule(args.config)
     nshell = m.nbas if m.aux_start < 0 else m.aux_start
     B = mc.Basis(m.atm, m.bas, m.env)
@@ -154,6 +165,18 @@ def main():
     dom_flops = bflops[dom]
     achieved = dom_flops / dom_t / 1e12
     total_flops = float(sum(bflops))
+    # HBM traffic of the dominant launch, when a committed counter profile
+    # (rocprofv3 FETCH_SIZE/WRITE_SIZE) matches this launch's kernel and grid
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "dominant_traffic.json")) as fh:
+            tr = json.load(fh)
+        b = buckets[dom]
+        grid = -(-b["tasks"] // (256 // b["team"])) * 256 if b["path"] == "fast" else None
+        if tuple(tr["cls"]) == tuple(b["cls"]) and tr["path"] == b["path"] and tr["grid_x"] == grid:
+            traffic = tr["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        traffic = None
     top = sorted(range(len(buckets)), key=lambda i: -btime[i])[:8]
     for i in top:
         b = buckets[i]
@@ -193,7 +216,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": f"class {buckets[dom]['cls']} team {buckets[dom]['team']} "
                       f"({buckets[dom]['path']}), "
                       f"{buckets[dom]['tasks']} quartets, {int(nprim[dom])} primitive quartets",
