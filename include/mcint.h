@@ -185,6 +185,26 @@ int mcint_plan_execute_bucket(mcint_plan *p, int bucket, double *out_dev, FINT *
  * array of nbuckets counts).  Synchronous. */
 int mcint_plan_count_primitives(mcint_plan *p, int64_t *per_bucket);
 
+/* Nuclear gradient digest of an int2e_ip1_sph plan whose list holds every
+ * (i, j, k>=l) shell tuple: adds to grad_dev[3*atom+x] (device, natm*3)
+ *   dE/dA_x = sum_{i on A} sum_{jkl} (d/dr_x i j|kl) [-2 D_ij D_kl + D_ik D_jl]
+ * for E = 1/2 sum D_ij D_kl (ij|kl) - 1/4 sum D_ik D_jl (ij|kl), without
+ * storing the derivative integrals.  dm_dev: nao x nao spherical density
+ * (row-major, symmetric).  Asynchronous on hip_stream. */
+int mcint_plan_execute_grad(mcint_plan *p, const double *dm_dev, int nao, double *grad_dev,
+                            void *hip_stream);
+
+/* Coulomb / exchange matrices from computed int2e_sph blocks: for every
+ * quartet q of shls_dev (4 ints each, device) with its block at
+ * eri_dev + off_dev[q], and every distinct permutation (pq|rs) of it,
+ *   vj[p,q] += (pq|rs) D[r,s],   vk[p,r] += (pq|rs) D[q,s].
+ * Pass the unique quartets (i>=j, k>=l, ij>=kl) once each to get the full
+ * J = sum_rs (pq|rs) D_rs and K = sum_qs (pq|rs) D_qs.  vj_dev / vk_dev
+ * (nao x nao, row-major, nullable) are accumulated into, not zeroed. */
+int mcint_jk_digest(const FINT *shls_dev, const int64_t *off_dev, size_t nq, const double *eri_dev,
+                    const mcint_basis *b, const double *dm_dev, double *vj_dev, double *vk_dev,
+                    void *hip_stream);
+
 /* Schwarz bounds q[i*nbas+j] = sqrt(max |(ij|ij)|) over the spherical
  * block of every shell pair among the first nshell shells (nshell <= 0:
  * all).  q is a host array of nbas*nbas doubles. */

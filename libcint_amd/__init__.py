@@ -85,6 +85,10 @@ def lib():
     L.mcint_plan_execute_bucket.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.mcint_plan_count_primitives.restype = ctypes.c_int
     L.mcint_plan_count_primitives.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_execute_grad.restype = ctypes.c_int
+    L.mcint_plan_execute_grad.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    L.mcint_jk_digest.restype = ctypes.c_int
+    L.mcint_jk_digest.argtypes = [vp, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp]
     L.mcint_schwarz.restype = ctypes.c_int
     L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
     L.mcint_last_error.restype = ctypes.c_char_p

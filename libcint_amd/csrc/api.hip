@@ -82,6 +82,7 @@ struct mcint_basis {
   std::vector<int> vfirst;  // first virtual shell of each real shell
   VShell* vsh_d = nullptr;
   double* env_d = nullptr;
+  int* ao_loc_d = nullptr;  // spherical AO offsets of the real shells, nbas+1
   double expc4 = EXPCUTOFF, expc3 = EXPCUTOFF, omega = 0;
   int device = 0;
 };
@@ -122,6 +123,9 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
     b->expc3 = std::max(MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]);      // src/g3c2e.c:54-58
   }
   b->omega = env[PTR_RANGE_OMEGA];
+  std::vector<int> ao_loc(nbas + 1, 0);
+  for (int s = 0; s < nbas; ++s)
+    ao_loc[s + 1] = ao_loc[s] + nsph(bas[(size_t)s * BAS_SLOTS + ANG_OF]) * bas[(size_t)s * BAS_SLOTS + NCTR_OF];
   for (int s = 0; s < nbas; ++s) {
     const int* bs = bas + (size_t)s * BAS_SLOTS;
     b->vfirst.push_back((int)b->vsh.size());
@@ -141,6 +145,8 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
       v.dsph = nsph(l) * bs[NCTR_OF];
       v.dcart = ncart(l) * bs[NCTR_OF];
       v.shell = s;
+      v.atom = bs[ATOM_OF];
+      v.ao = ao_loc[s] + c * nsph(l);
       b->vsh.push_back(v);
     }
   }
@@ -153,6 +159,9 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = dalloc(&b->vsh_d, b->vsh.size());
   if (e == hipSuccess) e = dalloc(&b->env_d, b->env.size());
+  if (e == hipSuccess) e = dalloc(&b->ao_loc_d, ao_loc.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->ao_loc_d, ao_loc.data(), ao_loc.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(b->vsh_d, b->vsh.data(), b->vsh.size() * sizeof(VShell), hipMemcpyHostToDevice);
   if (e == hipSuccess)
@@ -161,6 +170,7 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
     set_err(std::string("mcint_basis_create: ") + hipGetErrorString(e));
     dfree(b->vsh_d);
     dfree(b->env_d);
+    dfree(b->ao_loc_d);
     delete b;
     return MCINT_ERR_HIP;
   }
@@ -172,6 +182,7 @@ extern "C" void mcint_basis_destroy(mcint_basis* b) {
   if (!b) return;
   dfree(b->vsh_d);
   dfree(b->env_d);
+  dfree(b->ao_loc_d);
   delete b;
 }
 
@@ -547,6 +558,9 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.team = bk.team;
   A.t0 = bk.t0;
   A.t1 = bk.t1;
+  A.dm = nullptr;
+  A.grad = nullptr;
+  A.nao = 0;
   return A;
 }
 
@@ -579,6 +593,55 @@ static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t strea
     const int st = bucket_launch(p, bk, out, not0, stream);
     if (st) return st;
   }
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int nao,
+                                       double* grad_dev, void* hip_stream) {
+  if (!p || !dm_dev || !grad_dev || nao <= 0) {
+    set_err("mcint_plan_execute_grad: null argument");
+    return MCINT_ERR_ARG;
+  }
+  if (p->intor != MCINT_INT2E_IP1_SPH) {
+    set_err("mcint_plan_execute_grad: the plan must be int2e_ip1_sph");
+    return MCINT_ERR_ARG;
+  }
+  const int nao_b = CINTtot_cgto_spheric(p->b->bas.data(), p->b->nbas);
+  if (nao != nao_b) {
+    set_err("mcint_plan_execute_grad: density dimension does not match the basis");
+    return MCINT_ERR_ARG;
+  }
+  hipStream_t stream = (hipStream_t)hip_stream;
+  const BasisView B = plan_view(p);
+  for (const Bucket& bk : p->buckets) {
+    if (bk.mode != 0) {
+      set_err("mcint_plan_execute_grad: internal: derivative bucket on a non-generic path");
+      return MCINT_ERR_UNSUPPORTED;
+    }
+    BucketArgs A = bucket_args(p, bk, nullptr, nullptr);
+    A.dm = dm_dev;
+    A.grad = grad_dev;
+    A.nao = nao;
+    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
+    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+      A.t0 = t;
+      A.t1 = std::min(bk.t1, t + chunk);
+      HIPCHK(launch_generic(B, A, stream));
+    }
+  }
+  return MCINT_OK;
+}
+
+extern "C" int mcint_jk_digest(const FINT* shls_dev, const int64_t* off_dev, size_t nq,
+                               const double* eri_dev, const mcint_basis* b, const double* dm_dev,
+                               double* vj_dev, double* vk_dev, void* hip_stream) {
+  if (!b || (nq && (!shls_dev || !off_dev || !eri_dev)) || !dm_dev) {
+    set_err("mcint_jk_digest: null argument");
+    return MCINT_ERR_ARG;
+  }
+  const int nao = CINTtot_cgto_spheric(b->bas.data(), b->nbas);
+  HIPCHK(launch_jk_digest(shls_dev, off_dev, (int64_t)nq, eri_dev, b->ao_loc_d, dm_dev, nao, vj_dev,
+                          vk_dev, (hipStream_t)hip_stream));
   return MCINT_OK;
 }
 

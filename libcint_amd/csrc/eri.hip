@@ -324,6 +324,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
 
   if (!any) {
+    if (A.dm) return;  // nothing to contract
     const int n = G.ncomp * G.nAo * G.nBo * G.nCo * (three ? 1 : G.nDo);
     for (int i = lane; i < n; i += GEN_LANES) {
       int k = i;
@@ -440,10 +441,14 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
     H3 = X + G.oH2;
     n_c = G.nF;  // = ncart(lc)
   }
-  // pass 4: ket Cartesian -> spherical and the store
+  // pass 4: ket Cartesian -> spherical, then the store (or, for a gradient
+  // digest, the contraction with the density)
   {
     const int nD = three ? 1 : G.nDo;
     const int n = G.ncomp * G.nAo * G.nBo * G.nCo * nD;
+    const bool digest = A.dm != nullptr;
+    double gpart[3] = {0.0, 0.0, 0.0};
+    const double fkl = (!three && s2.shell != s3.shell) ? 2.0 : 1.0;
     for (int i = lane; i < n; i += GEN_LANES) {
       int k = i;
       const int ao = k % G.nAo; k /= G.nAo;
@@ -473,7 +478,26 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
           v += cc[c] * s;
         }
       }
-      out[comp * OS.comp + ao * OS.s[0] + bo * OS.s[1] + co * OS.s[2] + (three ? 0 : dd * OS.s[3])] = v;
+      if (digest) {
+        // dE/dA = sum (d/dr i j|kl) [-2 D_ij D_kl + (D_ik D_jl + D_il D_jk)/2], times 2
+        // when k and l are different shells (the list holds k >= l only)
+        const int64_t N = A.nao;
+        const int64_t P = s0.ao + ao, Q = s1.ao + bo, R = s2.ao + co, S = s3.ao + dd;
+        const double* D = A.dm;
+        const double w = fkl * (-2.0 * D[P * N + Q] * D[R * N + S] +
+                                0.5 * (D[P * N + R] * D[Q * N + S] + D[P * N + S] * D[Q * N + R]));
+        gpart[comp] += v * w;
+      } else {
+        out[comp * OS.comp + ao * OS.s[0] + bo * OS.s[1] + co * OS.s[2] +
+            (three ? 0 : dd * OS.s[3])] = v;
+      }
+    }
+    if (digest) {
+      for (int c = 0; c < 3; ++c) {
+        double g = gpart[c];
+        for (int off = 1; off < GEN_LANES; off <<= 1) g += __shfl_xor(g, off);
+        if (lane == 0) atomicAdd(A.grad + 3 * s0.atom + c, g);
+      }
     }
   }
 }

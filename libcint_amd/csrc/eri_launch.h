@@ -82,6 +82,10 @@ hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned 
 
 hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
+hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, const double* eri,
+                            const int* ao_loc, const double* D, int nao, double* J, double* K,
+                            hipStream_t stream);
+
 // Register-resident kernels for the classes listed in eri_fast.h:
 // mode 1 = one kernel with an in-register epilogue, 2 = chunk kernels plus
 // an LDS epilogue (scratch of chunked_block_doubles() per task), 0 = none.

@@ -34,6 +34,8 @@ struct VShell {
   int dsph;        // output width of the real shell, spherical: (2l+1) * nctr
   int dcart;       // output width of the real shell, Cartesian
   int shell;       // index of the real shell
+  int atom;        // atom index (gradient accumulation)
+  int ao;          // first spherical AO of this column (CINTshells_spheric_offset + c*(2l+1))
 };
 
 // Primitive-pair data of an ordered virtual-shell pair, primitive i fastest.
@@ -82,6 +84,11 @@ struct BucketArgs {
   int sph;
   int perm;              // canonical axis A maps to original axis (perm >> 2A) & 3
   int team;              // lanes per task (fast kernels)
+  // gradient digest (i-derivative plans): instead of storing the block,
+  // contract it with the density and add to grad[atom of shell i][x]
+  const double* dm;      // nao x nao density, row-major (nullable: store mode)
+  double* grad;          // natm x 3
+  int nao;
 };
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }

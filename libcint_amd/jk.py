@@ -1,0 +1,129 @@
+"""Coulomb/exchange matrices, the two-electron energy and its nuclear gradient.
+
+The caller-side layer of the integral path (not part of libcint; PySCF's
+``_vhf`` plays this role for the reference): on the device,
+
+    J_pq = sum_rs (pq|rs) D_rs,     K_pr = sum_qs (pq|rs) D_qs,
+    E    = 1/2 sum D_pq J_pq - 1/4 sum D_pr K_pr,
+    dE/dA_x = sum_{i on A} sum_jkl (d/dr_x i j|kl) [-2 D_ij D_kl + D_ik D_jl],
+
+where (d/dr_x i j|kl) is libcint's int2e_ip1 (src/autocode/grad2.c:19-68).
+The gradient is a digest of the derivative integrals computed on the fly
+(they are never stored).  ``TwoElectronEnergy`` exposes E(coords, D) as a
+torch.autograd.Function whose backward runs that digest.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import Basis, DeviceBasis, Plan, _check, lib
+from .synthetic import Molecule, unique_quartets
+
+
+def _ptr(t: torch.Tensor):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def nao_sph(basis: Basis) -> int:
+    return int(((2 * basis.bas[:, 1] + 1) * basis.bas[:, 3]).sum())
+
+
+def get_jk(basis: Basis, dm, nshell: int | None = None, chunk: int = 4_000_000,
+           quartets: np.ndarray | None = None, device="cuda"):
+    """J and K (torch float64 on the device) for a symmetric density dm."""
+    import ctypes
+    dev = torch.device(device)
+    nshell = basis.nbas if nshell is None else nshell
+    dm_t = torch.as_tensor(np.asarray(dm, dtype=np.float64), device=dev).contiguous()
+    nao = dm_t.shape[0]
+    vj = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
+    vk = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
+    q_all = unique_quartets(nshell) if quartets is None else np.asarray(quartets, np.int32)
+    db = DeviceBasis(basis)
+    for c0 in range(0, q_all.shape[0], chunk):
+        q = np.ascontiguousarray(q_all[c0:c0 + chunk])
+        plan = Plan(db, "int2e_sph", q)
+        eri = torch.empty(max(plan.out_size, 1), dtype=torch.float64, device=dev)
+        plan.execute(eri.data_ptr(), graph=False)
+        sizes = basis.block_sizes("int2e_sph", q)
+        off = np.zeros(q.shape[0], dtype=np.int64)
+        off[1:] = np.cumsum(sizes)[:-1]
+        q_d = torch.as_tensor(q, device=dev)
+        off_d = torch.as_tensor(off, device=dev)
+        _check(lib().mcint_jk_digest(_ptr(q_d), _ptr(off_d), ctypes.c_size_t(q.shape[0]),
+                                     _ptr(eri), db.handle, _ptr(dm_t), _ptr(vj), _ptr(vk), None),
+               "mcint_jk_digest")
+        torch.cuda.synchronize(dev)
+        plan.close()
+    db.close()
+    return vj, vk
+
+
+def energy(dm: torch.Tensor, vj: torch.Tensor, vk: torch.Tensor) -> torch.Tensor:
+    return 0.5 * torch.sum(dm * vj) - 0.25 * torch.sum(dm * vk)
+
+
+def gradient_quartets(nshell: int) -> np.ndarray:
+    """(i, j, k>=l) for all ordered i, j: the list the gradient digest needs."""
+    ii, jj = np.meshgrid(np.arange(nshell), np.arange(nshell), indexing="ij")
+    kk, ll = np.tril_indices(nshell)
+    nij, nkl = ii.size, kk.size
+    out = np.empty((nij * nkl, 4), dtype=np.int32)
+    out[:, 0] = np.repeat(ii.ravel(), nkl)
+    out[:, 1] = np.repeat(jj.ravel(), nkl)
+    out[:, 2] = np.tile(kk, nij)
+    out[:, 3] = np.tile(ll, nij)
+    return out
+
+
+def nuclear_gradient(basis: Basis, dm, natm: int, chunk: int = 8_000_000, device="cuda"):
+    """dE/dR (natm x 3, torch float64) of E = 1/2 D J - 1/4 D K."""
+    import ctypes
+    dev = torch.device(device)
+    dm_t = torch.as_tensor(np.asarray(dm, dtype=np.float64), device=dev).contiguous()
+    nao = dm_t.shape[0]
+    grad = torch.zeros((natm, 3), dtype=torch.float64, device=dev)
+    q_all = gradient_quartets(basis.nbas)
+    db = DeviceBasis(basis)
+    for c0 in range(0, q_all.shape[0], chunk):
+        plan = Plan(db, "int2e_ip1_sph", q_all[c0:c0 + chunk])
+        _check(lib().mcint_plan_execute_grad(plan.handle, _ptr(dm_t), nao, _ptr(grad), None),
+               "mcint_plan_execute_grad")
+        torch.cuda.synchronize(dev)
+        plan.close()
+    db.close()
+    return grad
+
+
+class TwoElectronEnergy(torch.autograd.Function):
+    """E(coords, D) = 1/2 sum D J - 1/4 sum D K for a fixed basis template.
+
+    forward: J/K on the device; backward: the nuclear-gradient digest for
+    coords and dE/dD = J - K/2 for the density.
+    """
+
+    @staticmethod
+    def forward(ctx, coords: torch.Tensor, dm: torch.Tensor, mol: Molecule):
+        m = mol.with_coords(coords.detach().cpu().numpy())
+        basis = Basis(m.atm, m.bas, m.env)
+        dm_d = dm.detach().to(dtype=torch.float64)
+        vj, vk = get_jk(basis, dm_d.cpu().numpy(), device=dm_d.device if dm_d.is_cuda else "cuda")
+        ctx.mol = m
+        ctx.save_for_backward(coords, dm_d)
+        ctx.vj, ctx.vk = vj, vk
+        e = energy(dm_d.to(vj.device), vj, vk)
+        return e.to(coords.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        coords, dm = ctx.saved_tensors
+        m = ctx.mol
+        gc = gd = None
+        if ctx.needs_input_grad[0]:
+            grad = nuclear_gradient(Basis(m.atm, m.bas, m.env), dm.cpu().numpy(), m.natm)
+            gc = (g.to(grad.device) * grad).to(coords.device, coords.dtype)
+        if ctx.needs_input_grad[1]:
+            gd = (g.to(ctx.vj.device) * (ctx.vj - 0.5 * ctx.vk)).to(dm.device, dm.dtype)
+        return gc, gd, None

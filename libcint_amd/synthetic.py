@@ -303,3 +303,13 @@ def unique_quartets(nbas: int) -> np.ndarray:
     out[:, 2] = ii[q]
     out[:, 3] = jj[q]
     return out
+
+
+def subset(mol: Molecule, natoms: int) -> Molecule:
+    """The first natoms atoms of a molecule with their shells (same env)."""
+    keep = mol.bas[:, ATOM_OF] < natoms
+    if mol.aux_start >= 0:
+        keep[mol.aux_start:] = False
+    return dataclasses.replace(mol, name=f"{mol.name}[:{natoms}]", atm=mol.atm[:natoms].copy(),
+                               bas=mol.bas[keep].copy(), symbols=mol.symbols[:natoms],
+                               coords=mol.coords[:natoms].copy(), aux_start=-1)
