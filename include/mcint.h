@@ -161,6 +161,16 @@ void mcint_basis_destroy(mcint_basis *b);
  * output must hold. */
 int mcint_plan_create(mcint_plan **p, mcint_basis *b, int intor, const FINT *shls, size_t nq,
                       const int64_t *out_offsets, int64_t *out_size);
+/* Mixed precision: as mcint_plan_create, plus fp32_flags[q] != 0 marks
+ * quartets whose recurrences and primitive contraction may run in FP32
+ * (roots, weights, pair factors and the stored result stay FP64).  Only
+ * classes with a single-chunk register kernel honour the flag; others stay
+ * FP64.  mcint_plan_fp32_tasks() returns how many tasks run in FP32. */
+int mcint_plan_create_mixed(mcint_plan **p, mcint_basis *b, int intor, const FINT *shls, size_t nq,
+                            const int64_t *out_offsets, const uint8_t *fp32_flags,
+                            int64_t *out_size);
+int64_t mcint_plan_fp32_tasks(const mcint_plan *p);
+
 /* Run the plan: out_dev and not0_dev (nullable) are device pointers. The
  * first call on given (out, not0, stream) captures a HIP graph that later
  * calls replay.  Asynchronous on hip_stream. */

@@ -70,6 +70,12 @@ def lib():
     L.mcint_plan_create.restype = ctypes.c_int
     L.mcint_plan_create.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ip, ctypes.c_size_t,
                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_create_mixed.restype = ctypes.c_int
+    L.mcint_plan_create_mixed.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ip, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_fp32_tasks.restype = ctypes.c_int64
+    L.mcint_plan_fp32_tasks.argtypes = [vp]
     L.mcint_plan_execute.restype = ctypes.c_int
     L.mcint_plan_execute.argtypes = [vp, vp, vp, vp]
     L.mcint_plan_execute_direct.restype = ctypes.c_int
@@ -219,7 +225,7 @@ class DeviceBasis:
 class Plan:
     """mcint_plan over a host list of shell tuples; outputs live on the device."""
 
-    def __init__(self, dbasis: DeviceBasis, name: str, shls, out_offsets=None):
+    def __init__(self, dbasis: DeviceBasis, name: str, shls, out_offsets=None, fp32_flags=None):
         self.dbasis = dbasis
         self.name = name
         shls = np.ascontiguousarray(shls, dtype=np.int32).reshape(-1, 4)
@@ -230,8 +236,17 @@ class Plan:
         if out_offsets is not None:
             self._off = np.ascontiguousarray(out_offsets, dtype=np.int64)
             offp = self._off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-        _check(lib().mcint_plan_create(ctypes.byref(h), dbasis.handle, INTOR_CODES[name], _ip(shls),
-                                       self.nq, offp, ctypes.byref(osz)), "mcint_plan_create")
+        if fp32_flags is None:
+            _check(lib().mcint_plan_create(ctypes.byref(h), dbasis.handle, INTOR_CODES[name],
+                                           _ip(shls), self.nq, offp, ctypes.byref(osz)),
+                   "mcint_plan_create")
+        else:
+            self._flags = np.ascontiguousarray(fp32_flags, dtype=np.uint8)
+            assert self._flags.size == self.nq
+            _check(lib().mcint_plan_create_mixed(ctypes.byref(h), dbasis.handle, INTOR_CODES[name],
+                                                 _ip(shls), self.nq, offp,
+                                                 self._flags.ctypes.data_as(ctypes.c_void_p),
+                                                 ctypes.byref(osz)), "mcint_plan_create_mixed")
         self.handle = h
         self.out_size = int(osz.value)
 
@@ -239,7 +254,8 @@ class Plan:
         n = (ctypes.c_int64 * 6)()
         _check(lib().mcint_plan_stats(self.handle, n), "mcint_plan_stats")
         return dict(tasks=n[0], buckets=n[1], fast_tasks=n[2], generic_tasks=n[3],
-                    prim_quartets=n[4], out_size=n[5])
+                    prim_quartets=n[4], out_size=n[5],
+                    fp32_tasks=int(lib().mcint_plan_fp32_tasks(self.handle)))
 
     def buckets(self) -> list:
         """Per bucket: dict(cls=(la,lb,lc,ld), perm, team, path, tasks)."""

@@ -194,6 +194,7 @@ namespace {
 struct Bucket {
   int la, lb, lc, ld, perm, team;
   int mode;  // 0 generic, 1 fast (single chunk), 2 chunked
+  int fp32;  // recurrences in FP32 (mode 1 only)
   int64_t t0, t1;
   int64_t sstride;  // generic scratch doubles per task
 };
@@ -225,6 +226,7 @@ struct mcint_plan {
   int* flags_d = nullptr;
   int64_t flags_size = 0;
   int64_t stats[6] = {0, 0, 0, 0, 0, 0};
+  int64_t stats_fp32 = 0;
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -249,8 +251,9 @@ static void plan_free(mcint_plan* p) {
 
 extern "C" void mcint_plan_destroy(mcint_plan* p) { plan_free(p); }
 
-extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
-                                 size_t nq, const int64_t* out_offsets, int64_t* out_size) {
+static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
+                            size_t nq, const int64_t* out_offsets, const uint8_t* fp32_flags,
+                            int64_t* out_size) {
   if (!out || !b || (!shls && nq)) {
     set_err("mcint_plan_create: null argument");
     return MCINT_ERR_ARG;
@@ -351,11 +354,14 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
     nprimq_total += nprimq;
     const int mode = fast_kernel_mode(kind, sph, la, lb, lc, ld);
     const int team = mode ? pick_team(nprimq) : 0;
+    // FP32 recurrences only where a single-chunk kernel exists; other
+    // flagged quartets stay FP64
+    const uint32_t lowp = (fp32_flags && fp32_flags[q] && mode == 1) ? 1u : 0u;
     int tl = 0;
     while ((1 << tl) < team) ++tl;
     const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |
                        ((uint32_t)ld << 9) | ((uint32_t)perm << 12) | ((uint32_t)tl << 20) |
-                       ((uint32_t)mode << 24);
+                       ((uint32_t)mode << 24) | (lowp << 26);
     // virtual expansion over contraction columns of the original axes
     int cidx[4] = {0, 0, 0, 0};
     for (;;) {
@@ -435,6 +441,8 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
     bk.perm = (k >> 12) & 255;
     bk.team = 1 << ((k >> 20) & 15);
     bk.mode = (k >> 24) & 3;
+    bk.fp32 = (k >> 26) & 1;
+    if (bk.fp32) p->stats_fp32 += bk.t1 - bk.t0;
     bk.t0 = cnt[i];
     bk.t1 = cnt[i + 1];
     bk.sstride = 0;
@@ -518,6 +526,19 @@ extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, co
   return MCINT_OK;
 }
 
+extern "C" int mcint_plan_create(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
+                                 size_t nq, const int64_t* out_offsets, int64_t* out_size) {
+  return plan_create_impl(out, b, intor, shls, nq, out_offsets, nullptr, out_size);
+}
+
+extern "C" int mcint_plan_create_mixed(mcint_plan** out, mcint_basis* b, int intor,
+                                       const FINT* shls, size_t nq, const int64_t* out_offsets,
+                                       const uint8_t* fp32_flags, int64_t* out_size) {
+  return plan_create_impl(out, b, intor, shls, nq, out_offsets, fp32_flags, out_size);
+}
+
+extern "C" int64_t mcint_plan_fp32_tasks(const mcint_plan* p) { return p ? p->stats_fp32 : 0; }
+
 extern "C" int mcint_plan_stats(const mcint_plan* p, int64_t* n) {
   if (!p || !n) return MCINT_ERR_ARG;
   for (int i = 0; i < 6; ++i) n[i] = p->stats[i];
@@ -556,6 +577,7 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.sph = p->sph;
   A.perm = bk.perm;
   A.team = bk.team;
+  A.fp32 = bk.fp32;
   A.t0 = bk.t0;
   A.t1 = bk.t1;
   A.dm = nullptr;

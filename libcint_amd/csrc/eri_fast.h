@@ -103,23 +103,23 @@ __device__ __forceinline__ void sfor(F&& f) {
 // 2D recurrence for one root and one Cartesian direction:
 //   V(e+1,0) = c00 V(e,0) + e b10 V(e-1,0)
 //   V(e,f+1) = c0p V(e,f) + f b01 V(e,f-1) + e b00 V(e-1,f)
-template <int E, int F>
-__device__ __forceinline__ void vrr2d(double (&V)[E + 1][F + 1], double v0, double c00, double c0p,
-                                      double b10, double b01, double b00) {
+template <int E, int F, typename R>
+__device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, R b10, R b01,
+                                      R b00) {
   V[0][0] = v0;
   sfor<E>([&](auto EI) {
     constexpr int e = decltype(EI)::value;
-    double v = c00 * V[e][0];
-    if constexpr (e > 0) v = fma((double)e * b10, V[e - 1][0], v);
+    R v = c00 * V[e][0];
+    if constexpr (e > 0) v = fma((R)e * b10, V[e - 1][0], v);
     V[e + 1][0] = v;
   });
   sfor<F>([&](auto FI) {
     constexpr int f = decltype(FI)::value;
     sfor<E + 1>([&](auto EI) {
       constexpr int e = decltype(EI)::value;
-      double v = c0p * V[e][f];
-      if constexpr (f > 0) v = fma((double)f * b01, V[e][f - 1], v);
-      if constexpr (e > 0) v = fma((double)e * b00, V[e - 1][f], v);
+      R v = c0p * V[e][f];
+      if constexpr (f > 0) v = fma((R)f * b01, V[e][f - 1], v);
+      if constexpr (e > 0) v = fma((R)e * b00, V[e - 1][f], v);
       V[e][f + 1] = v;
     });
   });
@@ -129,9 +129,9 @@ __device__ __forceinline__ void vrr2d(double (&V)[E + 1][F + 1], double v0, doub
 // index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
 // is the team's pair-emptiness verdict, 'any' whether this lane saw a
 // primitive quartet that passed libcint's two screening tests.
-template <int LA, int LB, int LC, int LD, int K0, int K1>
+template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
-                                           double (&acc)[K1 - K0], int& any, int& live) {
+                                           R (&acc)[K1 - K0], int& any, int& live) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
   const VShell& sa = Bv.vsh[tk.sh[0]];
@@ -167,7 +167,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   }
   live = live_ij && live_kl;
 
-  sfor<K1 - K0>([&](auto I) { acc[decltype(I)::value] = 0.0; });
+  sfor<K1 - K0>([&](auto I) { acc[decltype(I)::value] = R(0); });
   any = 0;
   if (!live) return;
   const int np = nij * nkl;
@@ -208,10 +208,12 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       const double b01 = b00 + tmp4 * aij;
       const double tmp2 = 2.0 * b00 * akl;
       const double tmp3 = 2.0 * b00 * aij;
-      double Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
-      vrr2d<E, F>(Vx, 1.0, pax - tmp2 * xpq, qcx + tmp3 * xpq, b10, b01, b00);
-      vrr2d<E, F>(Vy, 1.0, pay - tmp2 * ypq, qcy + tmp3 * ypq, b10, b01, b00);
-      vrr2d<E, F>(Vz, w[r] * fac1, paz - tmp2 * zpq, qcz + tmp3 * zpq, b10, b01, b00);
+      // coefficients in FP64; the recurrence and the products in R
+      R Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
+      vrr2d<E, F, R>(Vx, R(1), R(pax - tmp2 * xpq), R(qcx + tmp3 * xpq), R(b10), R(b01), R(b00));
+      vrr2d<E, F, R>(Vy, R(1), R(pay - tmp2 * ypq), R(qcy + tmp3 * ypq), R(b10), R(b01), R(b00));
+      vrr2d<E, F, R>(Vz, R(w[r] * fac1), R(paz - tmp2 * zpq), R(qcz + tmp3 * zpq), R(b10), R(b01),
+                     R(b00));
       sfor<K1 - K0>([&](auto KK) {
         constexpr int k = K0 + decltype(KK)::value;
         constexpr int i = k / NF, j = k % NF;
@@ -244,7 +246,7 @@ __device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& t
   st3 = so[(perm >> 6) & 3];
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R = double>
 __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int NE = C::NE, NF = C::NF;
@@ -260,9 +262,9 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   const VShell& sd = Bv.vsh[tk.sh[3]];
   const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
-  double acc[NE * NF];
+  R acc[NE * NF];
   int any, live;
-  accumulate<LA, LB, LC, LD, 0, NE * NF>(Bv, tk, T, tl, acc, any, live);
+  accumulate<LA, LB, LC, LD, 0, NE * NF, R>(Bv, tk, T, tl, acc, any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -292,45 +294,45 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
     return;
   }
   // powers of A-B and C-D
-  double abp[3][LB + 1], cdp[3][LD + 1];
+  R abp[3][LB + 1], cdp[3][LD + 1];
   {
-    const double ab[3] = {Ax - sb.r[0], Ay - sb.r[1], Az - sb.r[2]};
-    const double cd[3] = {Cx - sd.r[0], Cy - sd.r[1], Cz - sd.r[2]};
+    const R ab[3] = {R(Ax - sb.r[0]), R(Ay - sb.r[1]), R(Az - sb.r[2])};
+    const R cd[3] = {R(Cx - sd.r[0]), R(Cy - sd.r[1]), R(Cz - sd.r[2])};
     sfor<3>([&](auto DI) {
       constexpr int d = decltype(DI)::value;
-      abp[d][0] = 1.0;
-      cdp[d][0] = 1.0;
+      abp[d][0] = R(1);
+      cdp[d][0] = R(1);
       sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
       sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
     });
   }
   // bra: (a,b| = sum_t C(b,t) (A-B)^(b-t) [a+t|, then spherical over a and b
-  double Ys[NAo * NBo * NF];
+  R Ys[NAo * NBo * NF];
   sfor<NF>([&](auto JJ) {
     constexpr int j = decltype(JJ)::value;
-    double Yc[NA * NB];
+    R Yc[NA * NB];
     sfor<NA * NB>([&](auto AB) {
       constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
       constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
       constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
-      double s = 0.0;
+      R s = R(0);
       sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
         constexpr int tt = decltype(TT)::value;
         constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
         constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
         constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
-        const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
-        s = fma(cb * pw, acc[e * NF + j], s);
+        const R pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
+        s = fma(R(cb) * pw, acc[e * NF + j], s);
       });
       Yc[a * NB + b] = s;
     });
     sfor<NAo * NBo>([&](auto OB) {
       constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
-      double s = 0.0;
+      R s = R(0);
       sfor<NA * NB>([&](auto AB) {
         constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
         constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
-        if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
+        if constexpr (cc != 0.0) s = fma(R(cc), Yc[a * NB + b], s);
       });
       Ys[(ao * NBo + bo) * NF + j] = s;
     });
@@ -339,19 +341,19 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   sfor<NAo * NBo>([&](auto RW) {
     constexpr int row = decltype(RW)::value;
     constexpr int ao = row / NBo, bo = row % NBo;
-    double Zc[NC * ND];
+    R Zc[NC * ND];
     sfor<NC * ND>([&](auto CD) {
       constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
       constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
       constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
-      double s = 0.0;
+      R s = R(0);
       sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
         constexpr int tt = decltype(TT)::value;
         constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
         constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
         constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
-        const double pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
-        s = fma(cb * pw, Ys[row * NF + f], s);
+        const R pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+        s = fma(R(cb) * pw, Ys[row * NF + f], s);
       });
       Zc[c * ND + d] = s;
     });
@@ -359,13 +361,13 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
       constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
       constexpr int k = (row * NCo + co) * NDo + dd;
       if ((k & (T - 1)) == tl) {
-        double s = 0.0;
+        R s = R(0);
         sfor<NC * ND>([&](auto CD) {
           constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
           constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
-          if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
+          if constexpr (cc != 0.0) s = fma(R(cc), Zc[c * ND + d], s);
         });
-        out[ao * st0 + bo * st1 + co * st2 + dd * st3] = s;
+        out[ao * st0 + bo * st1 + co * st2 + dd * st3] = (double)s;
       }
     });
   });
@@ -627,7 +629,10 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
     const int64_t n = A.t1 - A.t0;                                                         \
     const int64_t per = 256 / A.team;                                                      \
     const dim3 grid((unsigned)((n + per - 1) / per));                                      \
-    hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d>), grid, dim3(256), 0, stream, B, A);   \
+    if (A.fp32)                                                                            \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float>), grid, dim3(256), 0, stream, B, A); \
+    else                                                                                   \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double>), grid, dim3(256), 0, stream, B, A); \
     return hipGetLastError();                                                              \
   }
 #define MCINT_CHUNK_DEFINE(a, b, c, d)                                                     \

@@ -84,6 +84,7 @@ struct BucketArgs {
   int sph;
   int perm;              // canonical axis A maps to original axis (perm >> 2A) & 3
   int team;              // lanes per task (fast kernels)
+  int fp32;              // recurrences and contraction in FP32 (fast kernels)
   // gradient digest (i-derivative plans): instead of storing the block,
   // contract it with the density and add to grad[atom of shell i][x]
   const double* dm;      // nao x nao density, row-major (nullable: store mode)

@@ -30,21 +30,42 @@ def nao_sph(basis: Basis) -> int:
     return int(((2 * basis.bas[:, 1] + 1) * basis.bas[:, 3]).sum())
 
 
+def fp32_quartet_flags(db: DeviceBasis, quartets: np.ndarray, dm, threshold: float) -> np.ndarray:
+    """1 where sqrt((ij|ij)) sqrt((kl|kl)) max|D| < threshold (FP32 tier)."""
+    qs = db.schwarz()
+    dmax = float(np.abs(np.asarray(dm)).max())
+    bound = qs[quartets[:, 0], quartets[:, 1]] * qs[quartets[:, 2], quartets[:, 3]] * dmax
+    return (bound < threshold).astype(np.uint8)
+
+
 def get_jk(basis: Basis, dm, nshell: int | None = None, chunk: int = 4_000_000,
-           quartets: np.ndarray | None = None, device="cuda"):
-    """J and K (torch float64 on the device) for a symmetric density dm."""
+           quartets: np.ndarray | None = None, device="cuda", fp32_threshold: float = 0.0,
+           stats: dict | None = None):
+    """J and K (torch float64 on the device) for a symmetric density dm.
+
+    fp32_threshold > 0 selects the mixed-precision tier: quartets whose
+    Schwarz bound times max|D| is below it run their recurrences in FP32.
+    """
     import ctypes
     dev = torch.device(device)
     nshell = basis.nbas if nshell is None else nshell
-    dm_t = torch.as_tensor(np.asarray(dm, dtype=np.float64), device=dev).contiguous()
+    dm_np = np.asarray(dm, dtype=np.float64)
+    dm_t = torch.as_tensor(dm_np, device=dev).contiguous()
     nao = dm_t.shape[0]
     vj = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
     vk = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
     q_all = unique_quartets(nshell) if quartets is None else np.asarray(quartets, np.int32)
     db = DeviceBasis(basis)
+    flags_all = fp32_quartet_flags(db, q_all, dm_np, fp32_threshold) if fp32_threshold > 0 else None
     for c0 in range(0, q_all.shape[0], chunk):
         q = np.ascontiguousarray(q_all[c0:c0 + chunk])
-        plan = Plan(db, "int2e_sph", q)
+        flags = None if flags_all is None else flags_all[c0:c0 + chunk]
+        plan = Plan(db, "int2e_sph", q, fp32_flags=flags)
+        if stats is not None:
+            st = plan.stats()
+            for k in ("tasks", "fp32_tasks"):
+                stats[k] = stats.get(k, 0) + st[k]
+            stats["flagged"] = stats.get("flagged", 0) + (0 if flags is None else int(flags.sum()))
         eri = torch.empty(max(plan.out_size, 1), dtype=torch.float64, device=dev)
         plan.execute(eri.data_ptr(), graph=False)
         sizes = basis.block_sizes("int2e_sph", q)

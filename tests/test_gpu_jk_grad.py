@@ -145,3 +145,23 @@ def test_autograd_function_matches_digest():
     np.testing.assert_allclose(coords.grad.numpy(), g, rtol=0, atol=1e-12)
     vj, vk = jk.get_jk(mc.Basis(m.atm, m.bas, m.env), dm)
     np.testing.assert_allclose(dmt.grad.cpu().numpy(), (vj - 0.5 * vk).cpu().numpy(), atol=1e-12)
+
+
+def test_mixed_precision_jk(c0_setup):
+    """FP32 tier for Schwarz*max|D| < 1e-7: J, K and energy stay within the
+    stated errors of the all-FP64 result."""
+    import libcint_amd as mc
+    from libcint_amd import jk
+    m, eri, dm = c0_setup
+    B = mc.Basis(m.atm, m.bas, m.env)
+    vj, vk = jk.get_jk(B, dm)
+    st = {}
+    vj32, vk32 = jk.get_jk(B, dm, fp32_threshold=1e-7, stats=st)
+    _say("mixed J/K", st)
+    assert st["fp32_tasks"] > 0
+    dj = float((vj32 - vj).abs().max())
+    dk = float((vk32 - vk).abs().max())
+    de = float(jk.energy(vj.new_tensor(dm), vj32, vk32) - jk.energy(vj.new_tensor(dm), vj, vk))
+    _say(f"max|dJ| {dj:.2e} max|dK| {dk:.2e} dE {de:.2e}")
+    assert dj < 1e-8 and dk < 1e-8
+    assert abs(de) < 1e-9
