@@ -286,6 +286,14 @@ class Plan:
         _check(f(self.handle, ctypes.c_void_p(out_ptr), ctypes.c_void_p(not0_ptr or None),
                  ctypes.c_void_p(stream_ptr or None)), "mcint_plan_execute")
 
+    def execute_grad(self, dm_ptr: int, nao: int, grad_ptr: int, stream_ptr: int = 0):
+        """Gradient digest of an int2e_ip1_sph plan: grad (natm x 3, device)
+        += contraction of the derivative integrals with the density dm."""
+        _check(lib().mcint_plan_execute_grad(self.handle, ctypes.c_void_p(dm_ptr), int(nao),
+                                             ctypes.c_void_p(grad_ptr),
+                                             ctypes.c_void_p(stream_ptr or None)),
+               "mcint_plan_execute_grad")
+
     def close(self):
         if self.handle:
             lib().mcint_plan_destroy(self.handle)

@@ -202,6 +202,8 @@ struct Bucket {
 // Scratch bound of the generic and chunked paths (doubles); buckets larger
 // than the scratch are launched in slices.
 constexpr int64_t GEN_SCRATCH_DOUBLES = 64ll << 20;
+// copies of the gradient accumulators (spreads the digest's atomics)
+constexpr int GRAD_SLOTS = 64;
 
 int pick_team(int64_t nprim_quartets) {
   int t = 1;
@@ -225,6 +227,7 @@ struct mcint_plan {
   int64_t scratch_size = 0;
   int* flags_d = nullptr;
   int64_t flags_size = 0;
+  double* gslots_d = nullptr;  // gradient digest accumulators
   int64_t stats[6] = {0, 0, 0, 0, 0, 0};
   int64_t stats_fp32 = 0;
   // graph cache
@@ -246,6 +249,7 @@ static void plan_free(mcint_plan* p) {
   dfree(p->pp_d);
   dfree(p->scratch_d);
   dfree(p->flags_d);
+  dfree(p->gslots_d);
   delete p;
 }
 
@@ -447,7 +451,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     bk.t1 = cnt[i + 1];
     bk.sstride = 0;
     if (bk.mode == 2) {
-      const int64_t nef = chunked_block_doubles(bk.la, bk.lb, bk.lc, bk.ld);
+      const int64_t nef = chunked_block_doubles(kind, bk.la, bk.lb, bk.lc, bk.ld);
       bk.sstride = nef;
       const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(bk.t1 - bk.t0, GEN_SCRATCH_DOUBLES / nef));
       max_gen = std::max(max_gen, chunk * nef);
@@ -583,6 +587,8 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.dm = nullptr;
   A.grad = nullptr;
   A.nao = 0;
+  A.nslot = 1;
+  A.grad_stride = 0;
   return A;
 }
 
@@ -635,22 +641,33 @@ extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int 
   }
   hipStream_t stream = (hipStream_t)hip_stream;
   const BasisView B = plan_view(p);
+  // per-block partial sums go to GRAD_SLOTS copies of the accumulators
+  // (block % GRAD_SLOTS), reduced into grad_dev at the end
+  const int n = 3 * p->b->natm;
+  if (!p->gslots_d) HIPCHK(dalloc(&p->gslots_d, (size_t)GRAD_SLOTS * n));
+  HIPCHK(hipMemsetAsync(p->gslots_d, 0, sizeof(double) * GRAD_SLOTS * n, stream));
   for (const Bucket& bk : p->buckets) {
-    if (bk.mode != 0) {
-      set_err("mcint_plan_execute_grad: internal: derivative bucket on a non-generic path");
+    if (bk.mode == 1) {
+      set_err("mcint_plan_execute_grad: internal: derivative bucket on the single-kernel path");
       return MCINT_ERR_UNSUPPORTED;
     }
     BucketArgs A = bucket_args(p, bk, nullptr, nullptr);
     A.dm = dm_dev;
-    A.grad = grad_dev;
+    A.grad = p->gslots_d;
     A.nao = nao;
+    A.nslot = GRAD_SLOTS;
+    A.grad_stride = n;
     const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
     for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
       A.t0 = t;
       A.t1 = std::min(bk.t1, t + chunk);
-      HIPCHK(launch_generic(B, A, stream));
+      if (bk.mode == 2)
+        HIPCHK(launch_chunked(B, A, stream));
+      else
+        HIPCHK(launch_generic(B, A, stream));
     }
   }
+  HIPCHK(launch_grad_reduce(p->gslots_d, GRAD_SLOTS, n, grad_dev, stream));
   return MCINT_OK;
 }
 

@@ -493,10 +493,11 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
       }
     }
     if (digest) {
+      double* gs = A.grad + (int64_t)(blockIdx.x % A.nslot) * A.grad_stride + 3 * s0.atom;
       for (int c = 0; c < 3; ++c) {
         double g = gpart[c];
         for (int off = 1; off < GEN_LANES; off <<= 1) g += __shfl_xor(g, off);
-        if (lane == 0) atomicAdd(A.grad + 3 * s0.atom + c, g);
+        if (lane == 0) atomicAdd(gs + c, g);
       }
     }
   }

@@ -77,16 +77,28 @@ __host__ __device__ constexpr double c2sc(int l, int m, int k) {
   return l <= 1 ? (m == k ? 1.0 : 0.0) : ct::c2s_coef[ct::c2s_offset[l] + m * ((l + 1) * (l + 2) / 2) + k];
 }
 
-template <int LA, int LB, int LC, int LD>
+// Class constants.  DER = 0: plain ERIs, rows e = shells LA..LA+LB.
+// DER = 1: i-derivative (libcint's int2e_ip1, src/autocode/grad2.c:19-68):
+// the recurrence runs to E = LA+LB+1 and the rows are two sets, the raised
+// set (shells LA+1..E, each primitive weighted by 2 a_i) followed by the
+// lowered set (shells LA-1..E-2, absent for LA = 0), from which the
+// epilogue forms (d/dA_x a b| = a_x (a-1_x, b| - 2 a_i (a+1_x, b|.
+template <int LA, int LB, int LC, int LD, int DER = 0>
 struct FastClass {
-  static constexpr int E = LA + LB;
+  static constexpr int E = LA + LB + DER;
   static constexpr int F = LC + LD;
   static constexpr int NR = (E + F) / 2 + 1;
-  static constexpr int NE = csum(LA, E);
+  static constexpr int NEP = DER ? csum(LA + 1, E) : csum(LA, E);
+  static constexpr int NEM = (DER && LA > 0) ? csum(LA - 1, E - 2) : 0;
+  static constexpr int NE = NEP + NEM;
   static constexpr int NF = csum(LC, F);
   static constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
   static constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
   static constexpr int NAo = 2 * LA + 1, NBo = 2 * LB + 1, NCo = 2 * LC + 1, NDo = 2 * LD + 1;
+  // first shell of the list that row i belongs to, and its index there
+  static constexpr int row_lo(int i) { return DER ? (i < NEP ? LA + 1 : LA - 1) : LA; }
+  static constexpr int row_k(int i) { return i < NEP ? i : i - NEP; }
+  static constexpr bool row_scaled(int i) { return DER && i < NEP; }
 };
 
 // Compile-time loop: f(std::integral_constant<int, 0..N-1>) expanded in place,
@@ -129,10 +141,10 @@ __device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, 
 // index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
 // is the team's pair-emptiness verdict, 'any' whether this lane saw a
 // primitive quartet that passed libcint's two screening tests.
-template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double>
+template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
                                            R (&acc)[K1 - K0], int& any, int& live) {
-  using C = FastClass<LA, LB, LC, LD>;
+  using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
   const VShell& sa = Bv.vsh[tk.sh[0]];
   const VShell& sb = Bv.vsh[tk.sh[1]];
@@ -146,7 +158,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   const int64_t pcd = (int64_t)tk.sh[2] * Bv.nv + tk.sh[3];
   const PairPrim* __restrict__ pij = Bv.pp + Bv.pair_off[pab];
   const PairPrim* __restrict__ pkl = Bv.pp + Bv.pair_off[pcd];
-  const double ld_ij = Bv.pair_ld[pab] * (LA + LB);
+  const double ld_ij = Bv.pair_ld[pab] * (LA + LB + DER);
   const double ld_kl = Bv.pair_ld[pcd] * (LC + LD);
   const double Ecut = Bv.expcutoff_4c;
   // 2 pi^3 / sqrt(pi) times the s/p angular factors (src/g1e.c:565-572)
@@ -195,6 +207,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
     const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
     const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * Q.kab;
+    const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
     double u[NR], w[NR];
     rys_roots<NR>(a0 * rr, u, w);
     const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
@@ -217,9 +230,13 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       sfor<K1 - K0>([&](auto KK) {
         constexpr int k = K0 + decltype(KK)::value;
         constexpr int i = k / NF, j = k % NF;
-        constexpr int ex = cpx(LA, i), ey = cpy(LA, i), ez = cpz(LA, i);
+        constexpr int lo = C::row_lo(i), ik = C::row_k(i);
+        constexpr int ex = cpx(lo, ik), ey = cpy(lo, ik), ez = cpz(lo, ik);
         constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
-        acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
+        if constexpr (C::row_scaled(i))
+          acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz] * sai, acc[k - K0]);
+        else
+          acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
       });
     });
   }
@@ -382,9 +399,9 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
 // ---------------------------------------------------------------------
 constexpr int CHUNK_TARGET = 96;
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int DER = 0>
 struct ChunkCfg {
-  using C = FastClass<LA, LB, LC, LD>;
+  using C = FastClass<LA, LB, LC, LD, DER>;
   static constexpr int NEF = C::NE * C::NF;
   static constexpr int NCH = (NEF + CHUNK_TARGET - 1) / CHUNK_TARGET;
   static constexpr int W = (NEF + NCH - 1) / NCH;
@@ -392,9 +409,9 @@ struct ChunkCfg {
   static constexpr int k1(int c) { return (c + 1) * W < NEF ? (c + 1) * W : NEF; }
 };
 
-template <int LA, int LB, int LC, int LD, int CH>
+template <int LA, int LB, int LC, int LD, int CH, int DER = 0>
 __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs A) {
-  using G = ChunkCfg<LA, LB, LC, LD>;
+  using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
   const int T = A.team;
   const int tid = threadIdx.x;
@@ -404,7 +421,7 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   const Task tk = A.tasks[t];
   double acc[K1 - K0];
   int any, live;
-  accumulate<LA, LB, LC, LD, K0, K1>(Bv, tk, T, tl, acc, any, live);
+  accumulate<LA, LB, LC, LD, K0, K1, double, DER>(Bv, tk, T, tl, acc, any, live);
   for (int off = 1; off < T; off <<= 1) {
     sfor<K1 - K0>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -558,24 +575,238 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
   }
 }
 
-template <int LA, int LB, int LC, int LD, int CH>
+// ---------------------------------------------------------------------
+// Epilogue of the i-derivative chunks (int2e_ip1, src/autocode/grad2.c:19-68):
+// per task, in LDS,
+//   (d/dA_x a b| = a_x (a-1_x, b| - (a+1_x, b|_2a_i        (bra transfers
+//     on the lowered and the 2 a_i-weighted raised sets, then spherical a,b)
+//   |c d) <- |f0] transfer, spherical c,d;
+// then either the store of the three component blocks (comp outermost,
+// libcint's layout) or, with A.dm set, the gradient digest
+//   g_x += (d/dA_x a b|c d) fkl [-2 D_ab D_cd + (D_ac D_bd + D_ad D_bc)/2]
+// summed per block over runs of equal atoms into slot (block % nslot) of
+// the gradient accumulators.
+// ---------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+struct Ip1EpiCfg {
+  using C = FastClass<LA, LB, LC, LD, 1>;
+  static constexpr int NEF = C::NE * C::NF;
+  static constexpr int ROWS = 3 * C::NAo * C::NBo;
+  static constexpr int PER = NEF + ROWS * C::NF;  // doubles of LDS per task
+  static constexpr int WORK = C::NF > ROWS ? C::NF : ROWS;
+  static constexpr int TE = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
+  static constexpr int TPB0 = 256 / TE;
+  static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
+  static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
+  static constexpr int THREADS = TPB * TE;
+  static_assert(PER <= 6144, "i-derivative epilogue exceeds its LDS budget");
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD, 1>;
+  using P = Ip1EpiCfg<LA, LB, LC, LD>;
+  constexpr int NEP = C::NEP, NF = C::NF, NEF = P::NEF, TE = P::TE, ROWS = P::ROWS;
+  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
+  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  __shared__ double sm[P::TPB * P::PER];
+  __shared__ double s_g[P::TPB][3];
+  __shared__ int s_atom[P::TPB];
+  const int team = threadIdx.x / TE;
+  const int tl = threadIdx.x % TE;
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * P::TPB + team;
+  const bool valid = t < A.t1;
+  const bool digest = A.dm != nullptr;
+  double* X = sm + team * P::PER;
+  double* Ys = X + NEF;
+  int any = 0;
+  Task tk;
+  if (valid) {
+    tk = A.tasks[t];
+    const int64_t slot = t - A.t0;
+    any = A.flags[slot];
+    const double* src = A.scratch + slot * NEF;
+    if (any)
+      for (int k = tl; k < NEF; k += TE) X[k] = src[k];
+  }
+  __syncthreads();
+  if (valid && any) {
+    const VShell& sa = Bv.vsh[tk.sh[0]];
+    const VShell& sb = Bv.vsh[tk.sh[1]];
+    const double ab[3] = {sa.r[0] - sb.r[0], sa.r[1] - sb.r[1], sa.r[2] - sb.r[2]};
+    double abp[3][LB + 1];
+    sfor<3>([&](auto DI) {
+      constexpr int d = decltype(DI)::value;
+      abp[d][0] = 1.0;
+      sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
+    });
+    for (int f = tl; f < NF; f += TE) {
+      sfor<3>([&](auto DI) {
+        constexpr int x = decltype(DI)::value;
+        double Yc[NA * NB];
+        sfor<NA * NB>([&](auto AB) {
+          constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+          constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
+          constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
+          constexpr int ux = ax + (x == 0), uy = ay + (x == 1), uz = az + (x == 2);
+          constexpr int adir = x == 0 ? ax : (x == 1 ? ay : az);
+          double sp = 0.0, sl = 0.0;
+          sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
+            constexpr int tt = decltype(TT)::value;
+            constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
+            constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
+            const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
+            constexpr int ep = cindex(LA + 1, ux + tx, uy + ty, uz + tz);
+            sp = fma(cb * pw, X[ep * NF + f], sp);
+            if constexpr (adir > 0) {
+              constexpr int ex = ax - (x == 0) + tx, ey = ay - (x == 1) + ty, ez = az - (x == 2) + tz;
+              constexpr int em = NEP + cindex(LA - 1, ex, ey, ez);
+              sl = fma(cb * pw, X[em * NF + f], sl);
+            }
+          });
+          Yc[a * NB + b] = fma(double(adir), sl, -sp);
+        });
+        sfor<NAo * NBo>([&](auto OB) {
+          constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
+          double s = 0.0;
+          sfor<NA * NB>([&](auto AB) {
+            constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+            constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+            if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
+          });
+          Ys[((x * NAo + ao) * NBo + bo) * NF + f] = s;
+        });
+      });
+    }
+  }
+  __syncthreads();
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (valid) {
+    const VShell& sa = Bv.vsh[tk.sh[0]];
+    const VShell& sb = Bv.vsh[tk.sh[1]];
+    const VShell& sc = Bv.vsh[tk.sh[2]];
+    const VShell& sd = Bv.vsh[tk.sh[3]];
+    int64_t st0, st1, st2, st3;
+    canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
+    const int64_t stc = (int64_t)sa.dsph * sb.dsph * sc.dsph * sd.dsph;
+    double* __restrict__ out = A.out + tk.out;
+    if (!any) {
+      if (!digest)
+        for (int k = tl; k < 3 * NAo * NBo * NCo * NDo; k += TE) {
+          const int ao = k % NAo, bo = (k / NAo) % NBo, co = (k / (NAo * NBo)) % NCo;
+          const int dd = (k / (NAo * NBo * NCo)) % NDo, x = k / (NAo * NBo * NCo * NDo);
+          out[x * stc + ao * st0 + bo * st1 + co * st2 + dd * st3] = 0.0;
+        }
+    } else {
+      const double cd[3] = {sc.r[0] - sd.r[0], sc.r[1] - sd.r[1], sc.r[2] - sd.r[2]};
+      double cdp[3][LD + 1];
+      sfor<3>([&](auto DI) {
+        constexpr int d = decltype(DI)::value;
+        cdp[d][0] = 1.0;
+        sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
+      });
+      const double fkl = sc.shell != sd.shell ? 2.0 : 1.0;
+      const int64_t N = A.nao;
+      for (int row = tl; row < ROWS; row += TE) {
+        const int x = row / (NAo * NBo);
+        const int ao = (row / NBo) % NAo, bo = row % NBo;
+        const double* yr = Ys + row * NF;
+        double Zc[NC * ND];
+        sfor<NC * ND>([&](auto CD) {
+          constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+          constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
+          constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
+          double s = 0.0;
+          sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
+            constexpr int tt = decltype(TT)::value;
+            constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
+            constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
+            constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
+            const double pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+            s = fma(cb * pw, yr[f], s);
+          });
+          Zc[c * ND + d] = s;
+        });
+        if (digest) {
+          const double* __restrict__ D = A.dm;
+          const int64_t Pi = sa.ao + ao, Qi = sb.ao + bo;
+          const double* __restrict__ DP = D + Pi * N;
+          const double* __restrict__ DQ = D + Qi * N;
+          const double dpq = -2.0 * DP[Qi];
+          double acc = 0.0;
+          sfor<NCo * NDo>([&](auto OD) {
+            constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+            double s = 0.0;
+            sfor<NC * ND>([&](auto CD) {
+              constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+              constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+              if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
+            });
+            const int64_t Ri = sc.ao + co, Si = sd.ao + dd;
+            const double w = fma(dpq, D[Ri * N + Si], 0.5 * (DP[Ri] * DQ[Si] + DP[Si] * DQ[Ri]));
+            acc = fma(s, w, acc);
+          });
+          acc *= fkl;
+          if (x == 0) g0 += acc;
+          else if (x == 1) g1 += acc;
+          else g2 += acc;
+        } else {
+          double* orow = out + x * stc + ao * st0 + bo * st1;
+          sfor<NCo * NDo>([&](auto OD) {
+            constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+            double s = 0.0;
+            sfor<NC * ND>([&](auto CD) {
+              constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+              constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+              if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
+            });
+            orow[co * st2 + dd * st3] = s;
+          });
+        }
+      }
+    }
+  }
+  if (digest) {
+    for (int off = 1; off < TE; off <<= 1) {
+      g0 += __shfl_xor(g0, off);
+      g1 += __shfl_xor(g1, off);
+      g2 += __shfl_xor(g2, off);
+    }
+    if (tl == 0) {
+      s_g[team][0] = g0;
+      s_g[team][1] = g1;
+      s_g[team][2] = g2;
+      s_atom[team] = (valid && any) ? Bv.vsh[tk.sh[0]].atom : -1;
+    }
+    __syncthreads();
+    grad_block_flush(A, s_g, s_atom, P::TPB);
+  }
+}
+
+template <int LA, int LB, int LC, int LD, int CH, int DER>
 hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t stream, dim3 grid) {
-  hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH>), grid, dim3(256), 0, stream, B, A);
+  hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER>), grid, dim3(256), 0, stream, B, A);
   return hipGetLastError();
 }
 
-template <int LA, int LB, int LC, int LD, int... CHs>
+template <int LA, int LB, int LC, int LD, int DER, int... CHs>
 hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t stream,
                          std::integer_sequence<int, CHs...>) {
   const int64_t n = A.t1 - A.t0;
   const int64_t per = 256 / A.team;
   const dim3 grid((unsigned)((n + per - 1) / per));
   hipError_t e = hipSuccess;
-  ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs>(B, A, stream, grid) : e)), ...);
+  ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
-  using P = EpiCfg<LA, LB, LC, LD>;
-  const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
-  hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+  if constexpr (DER == 0) {
+    using P = EpiCfg<LA, LB, LC, LD>;
+    const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+    hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+  } else {
+    using P = Ip1EpiCfg<LA, LB, LC, LD>;
+    const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+    hipLaunchKernelGGL((eri_ip1_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+  }
   return hipGetLastError();
 }
 
@@ -618,8 +849,33 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
   MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X) \
   MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X)
 
+// i-derivative (int2e_ip1) classes on the chunked path: shells up to d,
+// canonical LC >= LD (the derivative shell i stays first), one translation
+// unit per group (ip1_<g>.hip).
+#define MCINT_IP1_GROUP0(X) X(1, 2, 2, 2)
+#define MCINT_IP1_GROUP1(X) X(2, 1, 2, 2) X(0, 1, 1, 0) X(2, 0, 1, 0)
+#define MCINT_IP1_GROUP2(X) X(2, 2, 2, 1) X(0, 1, 1, 1) X(2, 0, 2, 0)
+#define MCINT_IP1_GROUP3(X) X(0, 2, 2, 2) X(0, 2, 1, 1) X(0, 0, 2, 0) X(1, 1, 0, 0)
+#define MCINT_IP1_GROUP4(X) X(1, 1, 2, 2) X(0, 2, 2, 0) X(0, 0, 2, 1) X(1, 1, 1, 0)
+#define MCINT_IP1_GROUP5(X) X(1, 2, 2, 1) X(1, 0, 2, 1) X(0, 0, 2, 2) X(1, 2, 0, 0)
+#define MCINT_IP1_GROUP6(X) X(2, 2, 1, 1) X(1, 1, 1, 1) X(0, 1, 0, 0) X(2, 0, 0, 0)
+#define MCINT_IP1_GROUP7(X) X(2, 1, 2, 1) X(0, 1, 2, 1) X(0, 0, 0, 0) X(0, 1, 2, 0) X(2, 1, 0, 0)
+#define MCINT_IP1_GROUP8(X) X(2, 0, 2, 2) X(2, 1, 2, 0) X(0, 0, 1, 0) X(0, 2, 0, 0) X(2, 2, 0, 0)
+#define MCINT_IP1_GROUP9(X) X(2, 2, 2, 0) X(2, 2, 1, 0) X(0, 0, 1, 1) X(0, 2, 1, 0)
+#define MCINT_IP1_GROUP10(X) X(0, 2, 2, 1) X(0, 1, 2, 2) X(1, 1, 2, 0) X(1, 0, 0, 0)
+#define MCINT_IP1_GROUP11(X) X(1, 1, 2, 1) X(1, 0, 2, 2) X(1, 2, 1, 0) X(1, 0, 1, 0)
+#define MCINT_IP1_GROUP12(X) X(1, 2, 1, 1) X(1, 2, 2, 0) X(2, 0, 1, 1) X(1, 0, 1, 1)
+#define MCINT_IP1_GROUP13(X) X(2, 1, 1, 1) X(2, 0, 2, 1) X(2, 1, 1, 0) X(1, 0, 2, 0)
+#define MCINT_IP1_CLASSES(X)                                                        \
+  MCINT_IP1_GROUP0(X) MCINT_IP1_GROUP1(X) MCINT_IP1_GROUP2(X) MCINT_IP1_GROUP3(X)   \
+  MCINT_IP1_GROUP4(X) MCINT_IP1_GROUP5(X) MCINT_IP1_GROUP6(X) MCINT_IP1_GROUP7(X)   \
+  MCINT_IP1_GROUP8(X) MCINT_IP1_GROUP9(X) MCINT_IP1_GROUP10(X) MCINT_IP1_GROUP11(X) \
+  MCINT_IP1_GROUP12(X) MCINT_IP1_GROUP13(X)
+
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
 // explicit instantiation of one class (used by fast_<g>.hip)
 #define MCINT_FAST_DEFINE(a, b, c, d)                                                      \
@@ -639,8 +895,15 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
   template <>                                                                              \
   hipError_t launch_chunked_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,     \
                                               hipStream_t stream) {                        \
-    return launch_chunks<a, b, c, d>(                                                      \
+    return launch_chunks<a, b, c, d, 0>(                                                   \
         B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d>::NCH>{});       \
+  }
+#define MCINT_IP1_DEFINE(a, b, c, d)                                                       \
+  template <>                                                                              \
+  hipError_t launch_ip1_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,         \
+                                          hipStream_t stream) {                            \
+    return launch_chunks<a, b, c, d, 1>(                                                   \
+        B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d, 1>::NCH>{});    \
   }
 
 #define MCINT_FAST_DECLARE(a, b, c, d)                                                     \
@@ -651,6 +914,10 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
   template <>                                                                              \
   hipError_t launch_chunked_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,     \
                                               hipStream_t stream);
+#define MCINT_IP1_DECLARE(a, b, c, d)                                                      \
+  template <>                                                                              \
+  hipError_t launch_ip1_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,         \
+                                          hipStream_t stream);
 
 // dispatch functions (expanded once, in eri.hip)
 #define MC_FAST_HAS(a, b, c, d) \
@@ -663,16 +930,30 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
   if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_chunked_class<a, b, c, d>(B, A, stream);
 #define MC_CHUNK_NEF(a, b, c, d) \
   if (la == a && lb == b && lc == c && ld == d) return ChunkCfg<a, b, c, d>::NEF;
+#define MC_IP1_NEF(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return ChunkCfg<a, b, c, d, 1>::NEF;
+#define MC_IP1_LAUNCH(a, b, c, d) \
+  if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_ip1_class<a, b, c, d>(B, A, stream);
 #define MCINT_FAST_DISPATCH                                                                \
   MCINT_FAST_CLASSES(MCINT_FAST_DECLARE)                                                   \
   MCINT_CHUNK_CLASSES(MCINT_CHUNK_DECLARE)                                                  \
+  MCINT_IP1_CLASSES(MCINT_IP1_DECLARE)                                                     \
   int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld) {                \
-    if (kind != KIND_2E || !sph) return 0;                                                 \
+    if (!sph) return 0;                                                                    \
+    if (kind == KIND_2E_IP1) {                                                             \
+      MCINT_IP1_CLASSES(MC_CHUNK_HAS)                                                      \
+      return 0;                                                                            \
+    }                                                                                      \
+    if (kind != KIND_2E) return 0;                                                         \
     MCINT_FAST_CLASSES(MC_FAST_HAS)                                                        \
     MCINT_CHUNK_CLASSES(MC_CHUNK_HAS)                                                      \
     return 0;                                                                              \
   }                                                                                        \
-  int chunked_block_doubles(int la, int lb, int lc, int ld) {                              \
+  int chunked_block_doubles(int kind, int la, int lb, int lc, int ld) {                    \
+    if (kind == KIND_2E_IP1) {                                                             \
+      MCINT_IP1_CLASSES(MC_IP1_NEF)                                                        \
+      return 0;                                                                            \
+    }                                                                                      \
     MCINT_CHUNK_CLASSES(MC_CHUNK_NEF)                                                      \
     return 0;                                                                              \
   }                                                                                        \
@@ -686,6 +967,10 @@ hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_
   hipError_t launch_chunked(const BasisView& B, const BucketArgs& A, hipStream_t stream) { \
     if (A.t1 <= A.t0) return hipSuccess;                                                   \
     if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
+    if (A.kind == KIND_2E_IP1) {                                                           \
+      MCINT_IP1_CLASSES(MC_IP1_LAUNCH)                                                     \
+      return hipErrorNotSupported;                                                         \
+    }                                                                                      \
     MCINT_CHUNK_CLASSES(MC_CHUNK_LAUNCH)                                                   \
     return hipErrorNotSupported;                                                           \
   }

@@ -82,6 +82,9 @@ hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned 
 
 hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
+// grad[i] += sum over the nslot accumulator copies of acc[slot * n + i]
+hipError_t launch_grad_reduce(const double* acc, int nslot, int n, double* grad, hipStream_t stream);
+
 hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, const double* eri,
                             const int* ao_loc, const double* D, int nao, double* J, double* K,
                             hipStream_t stream);
@@ -90,7 +93,7 @@ hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, con
 // mode 1 = one kernel with an in-register epilogue, 2 = chunk kernels plus
 // an LDS epilogue (scratch of chunked_block_doubles() per task), 0 = none.
 int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld);
-int chunked_block_doubles(int la, int lb, int lc, int ld);
+int chunked_block_doubles(int kind, int la, int lb, int lc, int ld);
 int fast_team_lanes_max();
 hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 hipError_t launch_chunked(const BasisView& B, const BucketArgs& A, hipStream_t stream);

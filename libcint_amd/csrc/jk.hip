@@ -51,6 +51,22 @@ __global__ void jk_digest_kernel(const int* __restrict__ shls, const int64_t* __
   }
 }
 
+__global__ void grad_reduce_kernel(const double* __restrict__ acc, int nslot, int n,
+                                   double* __restrict__ grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < nslot; ++k) s += acc[(int64_t)k * n + i];
+  grad[i] += s;
+}
+
+hipError_t launch_grad_reduce(const double* acc, int nslot, int n, double* grad, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                     acc, nslot, n, grad);
+  return hipGetLastError();
+}
+
 hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, const double* eri,
                             const int* ao_loc, const double* D, int nao, double* J, double* K,
                             hipStream_t stream) {

@@ -88,8 +88,10 @@ struct BucketArgs {
   // gradient digest (i-derivative plans): instead of storing the block,
   // contract it with the density and add to grad[atom of shell i][x]
   const double* dm;      // nao x nao density, row-major (nullable: store mode)
-  double* grad;          // natm x 3
+  double* grad;          // nslot blocks of grad_stride (= 3 natm) accumulators
   int nao;
+  int nslot;             // gradient accumulator copies (spreads the atomics)
+  int grad_stride;
 };
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }

@@ -99,22 +99,43 @@ def gradient_quartets(nshell: int) -> np.ndarray:
     return out
 
 
-def nuclear_gradient(basis: Basis, dm, natm: int, chunk: int = 8_000_000, device="cuda"):
+class GradientPlan:
+    """Device plans of the ip1 list (i, j, k >= l) of a basis, reusable for
+    any density: ``run(dm)`` returns dE/dR (natm x 3) of E = 1/2 D J - 1/4 D K."""
+
+    def __init__(self, basis: Basis, natm: int, chunk: int = 64_000_000, device="cuda"):
+        self.dev = torch.device(device)
+        self.natm = natm
+        self.nao = nao_sph(basis)
+        self.db = DeviceBasis(basis)
+        q_all = gradient_quartets(basis.nbas)
+        self.nquartets = q_all.shape[0]
+        self.plans = [Plan(self.db, "int2e_ip1_sph", q_all[c0:c0 + chunk])
+                      for c0 in range(0, q_all.shape[0], chunk)]
+
+    def run(self, dm, stream_ptr: int = 0) -> torch.Tensor:
+        dm_t = torch.as_tensor(dm, dtype=torch.float64, device=self.dev).contiguous()
+        assert dm_t.shape == (self.nao, self.nao)
+        grad = torch.zeros((self.natm, 3), dtype=torch.float64, device=self.dev)
+        for plan in self.plans:
+            plan.execute_grad(dm_t.data_ptr(), self.nao, grad.data_ptr(), stream_ptr)
+        return grad
+
+    def close(self):
+        for p in self.plans:
+            p.close()
+        self.plans = []
+        self.db.close()
+
+
+def nuclear_gradient(basis: Basis, dm, natm: int, device="cuda"):
     """dE/dR (natm x 3, torch float64) of E = 1/2 D J - 1/4 D K."""
-    import ctypes
-    dev = torch.device(device)
-    dm_t = torch.as_tensor(np.asarray(dm, dtype=np.float64), device=dev).contiguous()
-    nao = dm_t.shape[0]
-    grad = torch.zeros((natm, 3), dtype=torch.float64, device=dev)
-    q_all = gradient_quartets(basis.nbas)
-    db = DeviceBasis(basis)
-    for c0 in range(0, q_all.shape[0], chunk):
-        plan = Plan(db, "int2e_ip1_sph", q_all[c0:c0 + chunk])
-        _check(lib().mcint_plan_execute_grad(plan.handle, _ptr(dm_t), nao, _ptr(grad), None),
-               "mcint_plan_execute_grad")
-        torch.cuda.synchronize(dev)
-        plan.close()
-    db.close()
+    gp = GradientPlan(basis, natm, device=device)
+    try:
+        grad = gp.run(np.asarray(dm, dtype=np.float64))
+        torch.cuda.synchronize(gp.dev)
+    finally:
+        gp.close()
     return grad
 
 
