@@ -215,6 +215,21 @@ int mcint_jk_digest(const FINT *shls_dev, const int64_t *off_dev, size_t nq, con
                     const mcint_basis *b, const double *dm_dev, double *vj_dev, double *vk_dev,
                     void *hip_stream);
 
+/* J/K plan over a list of unique quartets (i>=j, k>=l, ij>=kl, each once)
+ * whose spherical ERI blocks sit at out_offsets[q] (NULL: packed in list
+ * order, the layout of mcint_plan_create) of a device buffer.  Groups the
+ * quartets by bra pair and by ket pair once; mcint_jk_plan_execute then
+ * overwrites vj_dev / vk_dev (nao x nao row-major, nullable to skip) with
+ * J = sum_rs (pq|rs) D_rs and K = sum_qs (pq|rs) D_qs for the symmetric
+ * density dm_dev.  No libcint counterpart (PySCF's _vhf direct-SCF
+ * contraction plays this role for the reference). */
+typedef struct mcint_jk_plan mcint_jk_plan;
+int mcint_jk_plan_create(mcint_jk_plan **out, mcint_basis *b, const FINT *shls, size_t nq,
+                         const int64_t *out_offsets);
+int mcint_jk_plan_execute(mcint_jk_plan *p, const double *eri_dev, const double *dm_dev,
+                          double *vj_dev, double *vk_dev, void *hip_stream);
+void mcint_jk_plan_destroy(mcint_jk_plan *p);
+
 /* Schwarz bounds q[i*nbas+j] = sqrt(max |(ij|ij)|) over the spherical
  * block of every shell pair among the first nshell shells (nshell <= 0:
  * all).  q is a host array of nbas*nbas doubles. */

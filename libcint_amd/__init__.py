@@ -95,6 +95,12 @@ def lib():
     L.mcint_plan_execute_grad.argtypes = [vp, vp, ctypes.c_int, vp, vp]
     L.mcint_jk_digest.restype = ctypes.c_int
     L.mcint_jk_digest.argtypes = [vp, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp]
+    L.mcint_jk_plan_create.restype = ctypes.c_int
+    L.mcint_jk_plan_create.argtypes = [ctypes.POINTER(vp), vp, ip, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_jk_plan_execute.restype = ctypes.c_int
+    L.mcint_jk_plan_execute.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.mcint_jk_plan_destroy.argtypes = [vp]
     L.mcint_schwarz.restype = ctypes.c_int
     L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
     L.mcint_last_error.restype = ctypes.c_char_p

@@ -39,8 +39,19 @@ if os.environ.get("MCINT_RESOURCE_USAGE"):
     CFLAGS += ["-Rpass-analysis=kernel-resource-usage"]
 
 
-def _compile(src: str) -> str:
+def _headers_mtime() -> float:
+    names = [n for n in os.listdir(CSRC) if n.endswith((".h", ".inc"))]
+    names = [os.path.join(CSRC, n) for n in names]
+    names.append(os.path.join(HERE, "..", "include", "mcint.h"))
+    return max(os.path.getmtime(n) for n in names)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if incremental and os.path.exists(obj):
+        newest = max(_headers_mtime(), os.path.getmtime(os.path.join(CSRC, src)))
+        if os.path.getmtime(obj) > newest:
+            return obj
     cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     t = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -53,12 +64,14 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 8) -> str:
+def build(jobs: int = 8, incremental: bool = False) -> str:
+    """Compile every translation unit and link; incremental=True reuses
+    objects newer than their source and every header."""
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found")
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -71,6 +84,7 @@ def build(jobs: int = 8) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
     t0 = time.time()
-    print(build(a.jobs), f"({time.time() - t0:.1f}s)")
+    print(build(a.jobs, a.incremental), f"({time.time() - t0:.1f}s)")

@@ -684,6 +684,165 @@ extern "C" int mcint_jk_digest(const FINT* shls_dev, const int64_t* off_dev, siz
   return MCINT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// J/K plan: grouped digest of a unique-quartet ERI list
+// ---------------------------------------------------------------------------
+struct mcint_jk_plan {
+  mcint_basis* b = nullptr;
+  int nao = 0;
+  int64_t nq = 0;
+  int* shls_d = nullptr;
+  int64_t* off_d = nullptr;
+  bool grouped = false;
+  int* rows_q_d = nullptr;
+  int* rows_ptr_d = nullptr;
+  int nrows = 0;
+  size_t rows_lds = 0;
+  int* cols_q_d = nullptr;
+  int* cols_ptr_d = nullptr;
+  int ncols = 0;
+  double* jp_d = nullptr;
+  double* kp_d = nullptr;
+};
+
+static void jk_plan_free(mcint_jk_plan* j) {
+  if (!j) return;
+  dfree(j->shls_d);
+  dfree(j->off_d);
+  dfree(j->rows_q_d);
+  dfree(j->rows_ptr_d);
+  dfree(j->cols_q_d);
+  dfree(j->cols_ptr_d);
+  dfree(j->jp_d);
+  dfree(j->kp_d);
+  delete j;
+}
+
+extern "C" void mcint_jk_plan_destroy(mcint_jk_plan* j) { jk_plan_free(j); }
+
+// quartet indices grouped by a shell-pair key (stable), groups of at most
+// JK_GROUP_MAX quartets
+static void jk_groups(const std::vector<int64_t>& key, int64_t nkeys, std::vector<int>& order,
+                      std::vector<int>& ptr) {
+  constexpr int JK_GROUP_MAX = 8192;
+  const size_t nq = key.size();
+  order.resize(nq);
+  std::vector<int64_t> cnt((size_t)nkeys + 1, 0);
+  for (size_t q = 0; q < nq; ++q) cnt[key[q] + 1]++;
+  for (int64_t k = 0; k < nkeys; ++k) cnt[k + 1] += cnt[k];
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (size_t q = 0; q < nq; ++q) order[pos[key[q]]++] = (int)q;
+  ptr.clear();
+  for (int64_t k = 0; k < nkeys; ++k)
+    for (int64_t a = cnt[k]; a < cnt[k + 1]; a += JK_GROUP_MAX) ptr.push_back((int)a);
+  ptr.push_back((int)nq);
+}
+
+extern "C" int mcint_jk_plan_create(mcint_jk_plan** out, mcint_basis* b, const FINT* shls,
+                                    size_t nq, const int64_t* out_offsets) {
+  if (!out || !b || (!shls && nq)) {
+    set_err("mcint_jk_plan_create: null argument");
+    return MCINT_ERR_ARG;
+  }
+  *out = nullptr;
+  if (nq > (size_t)INT32_MAX) {
+    set_err("mcint_jk_plan_create: more than 2^31-1 quartets");
+    return MCINT_ERR_ARG;
+  }
+  const int nbas = b->nbas;
+  const int* bas = b->bas.data();
+  std::vector<int> ao(nbas + 1, 0);
+  int dmax = 0;
+  for (int i = 0; i < nbas; ++i) {
+    const int d = nsph(bas[i * BAS_SLOTS + ANG_OF]) * bas[i * BAS_SLOTS + NCTR_OF];
+    ao[i + 1] = ao[i] + d;
+    dmax = std::max(dmax, d);
+  }
+  auto* j = new mcint_jk_plan();
+  j->b = b;
+  j->nao = ao[nbas];
+  j->nq = (int64_t)nq;
+  // block offsets: given, or packed in list order (as mcint_plan_create)
+  std::vector<int64_t> off(nq, 0);
+  int64_t run = 0;
+  for (size_t q = 0; q < nq; ++q) {
+    const FINT* sq = shls + 4 * q;
+    int64_t blk = 1;
+    for (int a = 0; a < 4; ++a) {
+      if (sq[a] < 0 || sq[a] >= nbas) {
+        jk_plan_free(j);
+        set_err("mcint_jk_plan_create: shell index out of range");
+        return MCINT_ERR_ARG;
+      }
+      blk *= ao[sq[a] + 1] - ao[sq[a]];
+    }
+    off[q] = out_offsets ? out_offsets[q] : run;
+    run += blk;
+  }
+  const size_t lds = ((size_t)2 * dmax * j->nao + 256) * sizeof(double);
+  j->grouped = dmax * dmax <= 256 && lds <= 64 * 1024 && (int64_t)nbas * nbas <= (64ll << 20);
+  hipError_t e = dalloc(&j->shls_d, 4 * std::max<size_t>(nq, 1));
+  if (e == hipSuccess) e = dalloc(&j->off_d, std::max<size_t>(nq, 1));
+  if (e == hipSuccess && nq) e = hipMemcpy(j->shls_d, shls, 4 * nq * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nq) e = hipMemcpy(j->off_d, off.data(), nq * sizeof(int64_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && j->grouped) {
+    j->rows_lds = lds;
+    std::vector<int64_t> key(nq);
+    std::vector<int> order, ptr;
+    for (size_t q = 0; q < nq; ++q) key[q] = (int64_t)shls[4 * q] * nbas + shls[4 * q + 1];
+    jk_groups(key, (int64_t)nbas * nbas, order, ptr);
+    j->nrows = (int)ptr.size() - 1;
+    e = dalloc(&j->rows_q_d, std::max<size_t>(nq, 1));
+    if (e == hipSuccess) e = dalloc(&j->rows_ptr_d, ptr.size());
+    if (e == hipSuccess && nq) e = hipMemcpy(j->rows_q_d, order.data(), nq * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(j->rows_ptr_d, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice);
+    for (size_t q = 0; q < nq; ++q) key[q] = (int64_t)shls[4 * q + 2] * nbas + shls[4 * q + 3];
+    jk_groups(key, (int64_t)nbas * nbas, order, ptr);
+    j->ncols = (int)ptr.size() - 1;
+    if (e == hipSuccess) e = dalloc(&j->cols_q_d, std::max<size_t>(nq, 1));
+    if (e == hipSuccess) e = dalloc(&j->cols_ptr_d, ptr.size());
+    if (e == hipSuccess && nq) e = hipMemcpy(j->cols_q_d, order.data(), nq * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(j->cols_ptr_d, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&j->jp_d, (size_t)j->nao * j->nao);
+    if (e == hipSuccess) e = dalloc(&j->kp_d, (size_t)j->nao * j->nao);
+  }
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_jk_plan_create: ") + hipGetErrorString(e));
+    jk_plan_free(j);
+    return MCINT_ERR_HIP;
+  }
+  *out = j;
+  return MCINT_OK;
+}
+
+extern "C" int mcint_jk_plan_execute(mcint_jk_plan* j, const double* eri_dev, const double* dm_dev,
+                                     double* vj_dev, double* vk_dev, void* hip_stream) {
+  if (!j || !dm_dev || (j->nq && !eri_dev)) {
+    set_err("mcint_jk_plan_execute: null argument");
+    return MCINT_ERR_ARG;
+  }
+  hipStream_t stream = (hipStream_t)hip_stream;
+  const size_t nn = (size_t)j->nao * j->nao * sizeof(double);
+  if (!j->grouped) {
+    if (vj_dev) HIPCHK(hipMemsetAsync(vj_dev, 0, nn, stream));
+    if (vk_dev) HIPCHK(hipMemsetAsync(vk_dev, 0, nn, stream));
+    HIPCHK(launch_jk_digest(j->shls_d, j->off_d, j->nq, eri_dev, j->b->ao_loc_d, dm_dev, j->nao,
+                            vj_dev, vk_dev, stream));
+    return MCINT_OK;
+  }
+  double* Jp = vj_dev ? j->jp_d : nullptr;
+  double* Kp = vk_dev ? j->kp_d : nullptr;
+  if (Jp) HIPCHK(hipMemsetAsync(Jp, 0, nn, stream));
+  if (Kp) HIPCHK(hipMemsetAsync(Kp, 0, nn, stream));
+  if (Jp || Kp)
+    HIPCHK(launch_jk_grouped(j->shls_d, j->off_d, j->rows_q_d, j->rows_ptr_d, j->nrows, j->rows_lds,
+                             j->cols_q_d, j->cols_ptr_d, j->ncols, eri_dev, j->b->ao_loc_d, dm_dev,
+                             j->nao, Jp, Kp, stream));
+  if (Jp) HIPCHK(launch_symmetrize(Jp, vj_dev, j->nao, stream));
+  if (Kp) HIPCHK(launch_symmetrize(Kp, vk_dev, j->nao, stream));
+  return MCINT_OK;
+}
+
 extern "C" int mcint_plan_nbuckets(const mcint_plan* p) { return p ? (int)p->buckets.size() : 0; }
 
 extern "C" int mcint_plan_bucket_info(const mcint_plan* p, int i, int64_t* info) {

@@ -85,6 +85,15 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
 // grad[i] += sum over the nslot accumulator copies of acc[slot * n + i]
 hipError_t launch_grad_reduce(const double* acc, int nslot, int n, double* grad, hipStream_t stream);
 
+// Grouped J/K digest: row pass (groups of quartets sharing (i,j), LDS of
+// rows_lds bytes) and column pass (groups sharing (k,l)) into the half
+// matrices J', K' (either nullable); symmetrize then forms X + X^T.
+hipError_t launch_jk_grouped(const int* shls, const int64_t* off, const int* rows_q,
+                             const int* rows_ptr, int nrows, size_t rows_lds, const int* cols_q,
+                             const int* cols_ptr, int ncols, const double* eri, const int* ao_loc,
+                             const double* D, int nao, double* Jp, double* Kp, hipStream_t stream);
+hipError_t launch_symmetrize(const double* X, double* out, int n, hipStream_t stream);
+
 hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, const double* eri,
                             const int* ao_loc, const double* D, int nao, double* J, double* K,
                             hipStream_t stream);

@@ -31,54 +31,97 @@ def nao_sph(basis: Basis) -> int:
 
 
 def fp32_quartet_flags(db: DeviceBasis, quartets: np.ndarray, dm, threshold: float) -> np.ndarray:
-    """1 where sqrt((ij|ij)) sqrt((kl|kl)) max|D| < threshold (FP32 tier)."""
+    """1 where sqrt((ij|ij)) sqrt((kl|kl)) max|D| < threshold (FP32 tier);
+    dm may be the density or any array whose largest |entry| bounds it."""
     qs = db.schwarz()
     dmax = float(np.abs(np.asarray(dm)).max())
     bound = qs[quartets[:, 0], quartets[:, 1]] * qs[quartets[:, 2], quartets[:, 3]] * dmax
     return (bound < threshold).astype(np.uint8)
 
 
-def get_jk(basis: Basis, dm, nshell: int | None = None, chunk: int = 4_000_000,
-           quartets: np.ndarray | None = None, device="cuda", fp32_threshold: float = 0.0,
-           stats: dict | None = None):
+class JKBuilder:
+    """J and K of a basis for any symmetric density, on the device.
+
+    Holds the ERI plan of the unique quartets (optionally Schwarz-screened
+    and split into an FP32 tier), the device ERI buffer and the grouped
+    digest plan.  ``run(dm)`` recomputes the ERIs (direct mode) unless
+    ``recompute=False`` and the buffer is current, then digests them.
+    fp32_threshold > 0 selects the mixed-precision tier: quartets whose
+    Schwarz bound times dmax (a bound on |D|) is below it run their
+    recurrences in FP32.
+    """
+
+    def __init__(self, basis: Basis, quartets: np.ndarray | None = None, device="cuda",
+                 fp32_threshold: float = 0.0, dmax: float = 1.0):
+        import ctypes
+        self.dev = torch.device(device)
+        self.nao = nao_sph(basis)
+        self.db = DeviceBasis(basis)
+        q = unique_quartets(basis.nbas) if quartets is None else np.ascontiguousarray(quartets, np.int32)
+        self.quartets = q
+        flags = None
+        if fp32_threshold > 0:
+            flags = fp32_quartet_flags(self.db, q, np.array([dmax]), fp32_threshold)
+        self.plan = Plan(self.db, "int2e_sph", q, fp32_flags=flags)
+        self.eri = torch.empty(max(self.plan.out_size, 1), dtype=torch.float64, device=self.dev)
+        h = ctypes.c_void_p()
+        _check(lib().mcint_jk_plan_create(ctypes.byref(h), self.db.handle,
+                                          q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          q.shape[0], None), "mcint_jk_plan_create")
+        self.jk_handle = h
+        self.fresh = False
+
+    def stats(self) -> dict:
+        return self.plan.stats()
+
+    def compute_eri(self, stream_ptr: int = 0, graph: bool = False):
+        self.plan.execute(self.eri.data_ptr(), stream_ptr=stream_ptr, graph=graph)
+        self.fresh = True
+
+    def digest(self, dm, with_j=True, with_k=True, stream_ptr: int = 0):
+        dm_t = torch.as_tensor(dm, dtype=torch.float64, device=self.dev).contiguous()
+        vj = torch.empty((self.nao, self.nao), dtype=torch.float64, device=self.dev) if with_j else None
+        vk = torch.empty((self.nao, self.nao), dtype=torch.float64, device=self.dev) if with_k else None
+        _check(lib().mcint_jk_plan_execute(self.jk_handle, _ptr(self.eri), _ptr(dm_t),
+                                           _ptr(vj) if with_j else None,
+                                           _ptr(vk) if with_k else None,
+                                           stream_ptr or None), "mcint_jk_plan_execute")
+        return vj, vk
+
+    def run(self, dm, recompute: bool = True, stream_ptr: int = 0):
+        if recompute or not self.fresh:
+            self.compute_eri(stream_ptr)
+        return self.digest(dm, stream_ptr=stream_ptr)
+
+    def close(self):
+        if self.jk_handle:
+            lib().mcint_jk_plan_destroy(self.jk_handle)
+            self.jk_handle = None
+        self.plan.close()
+        self.db.close()
+
+
+def get_jk(basis: Basis, dm, nshell: int | None = None, quartets: np.ndarray | None = None,
+           device="cuda", fp32_threshold: float = 0.0, stats: dict | None = None):
     """J and K (torch float64 on the device) for a symmetric density dm.
 
     fp32_threshold > 0 selects the mixed-precision tier: quartets whose
     Schwarz bound times max|D| is below it run their recurrences in FP32.
     """
-    import ctypes
-    dev = torch.device(device)
-    nshell = basis.nbas if nshell is None else nshell
     dm_np = np.asarray(dm, dtype=np.float64)
-    dm_t = torch.as_tensor(dm_np, device=dev).contiguous()
-    nao = dm_t.shape[0]
-    vj = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
-    vk = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
-    q_all = unique_quartets(nshell) if quartets is None else np.asarray(quartets, np.int32)
-    db = DeviceBasis(basis)
-    flags_all = fp32_quartet_flags(db, q_all, dm_np, fp32_threshold) if fp32_threshold > 0 else None
-    for c0 in range(0, q_all.shape[0], chunk):
-        q = np.ascontiguousarray(q_all[c0:c0 + chunk])
-        flags = None if flags_all is None else flags_all[c0:c0 + chunk]
-        plan = Plan(db, "int2e_sph", q, fp32_flags=flags)
+    if quartets is None and nshell is not None:
+        quartets = unique_quartets(nshell)
+    jb = JKBuilder(basis, quartets, device=device, fp32_threshold=fp32_threshold,
+                   dmax=float(np.abs(dm_np).max()))
+    try:
         if stats is not None:
-            st = plan.stats()
+            st = jb.stats()
             for k in ("tasks", "fp32_tasks"):
                 stats[k] = stats.get(k, 0) + st[k]
-            stats["flagged"] = stats.get("flagged", 0) + (0 if flags is None else int(flags.sum()))
-        eri = torch.empty(max(plan.out_size, 1), dtype=torch.float64, device=dev)
-        plan.execute(eri.data_ptr(), graph=False)
-        sizes = basis.block_sizes("int2e_sph", q)
-        off = np.zeros(q.shape[0], dtype=np.int64)
-        off[1:] = np.cumsum(sizes)[:-1]
-        q_d = torch.as_tensor(q, device=dev)
-        off_d = torch.as_tensor(off, device=dev)
-        _check(lib().mcint_jk_digest(_ptr(q_d), _ptr(off_d), ctypes.c_size_t(q.shape[0]),
-                                     _ptr(eri), db.handle, _ptr(dm_t), _ptr(vj), _ptr(vk), None),
-               "mcint_jk_digest")
-        torch.cuda.synchronize(dev)
-        plan.close()
-    db.close()
+        vj, vk = jb.run(dm_np)
+        torch.cuda.synchronize(jb.dev)
+    finally:
+        jb.close()
     return vj, vk
 
 
