@@ -11,9 +11,11 @@ HBM).  Reported value: spherical ERIs per second (sum of di dj dk dl over the
 computed quartets / time), the definition of examples/time_c60.c:188,230.
 
 Also reported: the dominant bucket's algorithmic FLOP rate against MI355X's
-FP64 peak (HIP events on the plan's stream), and the CPU oracle (this
+FP64 peak (HIP events on the plan's stream), the CPU oracle (this
 repository's C restatement of libcint, OpenMP over quartets) on a bounded
-random sample of the same quartets.
+random sample of the same quartets, and ("gradient") the J/K build and the
+ip1 nuclear-gradient digest on the 20-atom C8H12 cluster timed against the
+plain FP64 int2e_sph pass over the same molecule.
 
     python bench.py [--steps K] [--warmup W] [--config c1]
 """
@@ -74,6 +76,76 @@ def cpu_baseline(basis_np, quartets, target_s=12.0, max_threads=16):
                        f"oracle/cint_oracle.c with tabulated roots, OpenMP dynamic schedule")
 
 
+def gradient_section(reps: int = 3) -> dict:
+    """The derivative and J/K legs on the 20-atom C8H12 cluster (DZ basis
+    with d on H): plain FP64 int2e_sph over all unique quartets, the J/K
+    build (ERIs + grouped digest, FP64 and with the FP32 tier), and the
+    nuclear-gradient digest over the ip1 list (i, j, k >= l), each timed on
+    device-resident data after one untimed run."""
+    import torch
+    import libcint_amd as mc
+    from libcint_amd import jk
+    from libcint_amd.synthetic import make_molecule, random_density, unique_quartets
+
+    m = make_molecule("c3")
+    B = mc.Basis(m.atm, m.bas, m.env)
+    dm = random_density(m.nao_sph(), seed=5)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    db = mc.DeviceBasis(B)
+    q = unique_quartets(m.nbas)
+    plan = mc.Plan(db, "int2e_sph", q)
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    t_plain, _ = timed(lambda: plan.execute(out.data_ptr(), graph=False))
+    plan.close()
+    del out
+    jb = jk.JKBuilder(B)
+    t_jk, (vj, vk) = timed(lambda: jb.run(dm))
+    dmt = torch.as_tensor(dm, device="cuda")
+    e64 = float(jk.energy(dmt, vj, vk))
+    jb.close()
+    jb32 = jk.JKBuilder(B, fp32_threshold=1e-7, dmax=float(np.abs(dm).max()))
+    fp32_tasks = jb32.stats()["fp32_tasks"]
+    t_jk32, (vj32, vk32) = timed(lambda: jb32.run(dm))
+    e32 = float(jk.energy(dmt, vj32, vk32))
+    dj = float((vj32 - vj).abs().max())
+    dk = float((vk32 - vk).abs().max())
+    jb32.close()
+    t0 = time.perf_counter()
+    gp = jk.GradientPlan(B, m.natm)
+    t_gplan = time.perf_counter() - t0
+    t_grad, g = timed(lambda: gp.run(dm))
+    gsum = float(g.sum(0).abs().max())
+    nq_grad = gp.nquartets
+    gp.close()
+    db.close()
+    return {
+        "workload": f"c3: {m.name}, {m.natm} atoms, {m.nbas} shells, {m.nao_sph()} AOs; "
+                    f"random density D = A A^T / N",
+        "plain_int2e_sph_ms": t_plain * 1e3,
+        "plain_unique_quartets": int(q.shape[0]),
+        "ip1_gradient_ms": t_grad * 1e3,
+        "ip1_gradient_quartets": int(nq_grad),
+        "ip1_gradient_vs_plain": t_grad / t_plain,
+        "gradient_plan_create_s": t_gplan,
+        "gradient_translational_residual": gsum,
+        "jk_fp64_ms": t_jk * 1e3,
+        "jk_mixed_ms": t_jk32 * 1e3,
+        "jk_mixed_fp32_quartets": int(fp32_tasks),
+        "jk_mixed_max_abs_dJ": dj,
+        "jk_mixed_max_abs_dK": dk,
+        "jk_mixed_energy_error": e32 - e64,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,12 +155,15 @@ def main():
     ap.add_argument("--schwarz", type=float, default=1e-12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the J/K and nuclear-gradient timings on the C8H12 cluster")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the quartet list "
-              "would be split by shell pair across GPUs, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a multi-GPU run "
+              "would split the quartet list by shell pair, one shard per GPU, and all-reduce "
+              "J, K and the gradient", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -224,6 +299,12 @@ def main():
             "flops_per_launch": dom_flops,
         },
     }
+    del out
+    plan.close()
+    if not args.no_gradient:
+        log("gradient / J/K legs ...")
+        result["gradient"] = gradient_section()
+        log(f"gradient: {result['gradient']}")
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(B, quart, target_s=args.cpu_seconds)

@@ -22,7 +22,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmcint.so")
 
 # longest compiles first
-SOURCES = ([f"ip1_{g}.hip" for g in range(14)]
+SOURCES = ([f"ip1_{g}.hip" for g in (14, *range(14))]
            + [f"chunk_{g}.hip" for g in (10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0)]
            + [f"fast_{g}.hip" for g in (7, 3, 8, 5, 6, 4, 2, 1, 0)] + ["eri.hip", "jk.hip", "api.hip"])
 

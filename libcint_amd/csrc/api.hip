@@ -268,14 +268,6 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     set_err("mcint_plan_create: unknown integral kind");
     return MCINT_ERR_ARG;
   }
-  if (kind != KIND_3C2E && b->omega < 0) {
-    set_err("mcint_plan_create: short-range (omega < 0) 4-centre integrals are not supported");
-    return MCINT_ERR_UNSUPPORTED;
-  }
-  if (kind == KIND_3C2E && b->omega != 0) {
-    set_err("mcint_plan_create: range-separated 3-centre integrals are not supported");
-    return MCINT_ERR_UNSUPPORTED;
-  }
   const int sph = intor_sph(intor);
   const int ncomp = kind == KIND_2E_IP1 ? 3 : 1;
   const int naxes = kind == KIND_3C2E ? 3 : 4;
@@ -356,7 +348,8 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     const int la = l[L[0]], lb = l[L[1]], lc = l[L[2]], ld = kind == KIND_3C2E ? 0 : l[L[3]];
     const int64_t nprimq = (int64_t)np[0] * np[1] * np[2] * (naxes == 4 ? np[3] : 1);
     nprimq_total += nprimq;
-    const int mode = fast_kernel_mode(kind, sph, la, lb, lc, ld);
+    // range-separated operators run on the generic kernel
+    const int mode = b->omega != 0 ? 0 : fast_kernel_mode(kind, sph, la, lb, lc, ld);
     const int team = mode ? pick_team(nprimq) : 0;
     // FP32 recurrences only where a single-chunk kernel exists; other
     // flagged quartets stay FP64
@@ -459,7 +452,10 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       p->stats[2] += bk.t1 - bk.t0;
     } else if (bk.mode == 0) {
       const GenLayout G = gen_layout(kind, sph, bk.la, bk.lb, bk.lc, bk.ld);
-      if (G.nE > 220 || G.nF > 165) {
+      const int nr = (bk.la + bk.lb + bk.lc + (kind == KIND_3C2E ? 0 : bk.ld) +
+                      (kind == KIND_2E_IP1 ? 1 : 0)) / 2 + 1;
+      const int nrt = b->omega < 0 ? 2 * nr : nr;
+      if (G.nE > 220 || G.nF > 165 || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
         plan_free(p);
         set_err("mcint_plan_create: angular momentum beyond the generic kernel's tables");
         return MCINT_ERR_UNSUPPORTED;

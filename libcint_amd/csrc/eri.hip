@@ -100,7 +100,6 @@ __device__ __forceinline__ OutStrides out_strides(const VShell* vsh, const Task&
 // per-task scratch block; used for every class without a fast kernel.
 // ---------------------------------------------------------------------
 constexpr int GEN_LANES = 64;
-constexpr int GEN_VCAP = 3 * NRYS_MAX * 90;  // e+f <= 17 bounds (E+1)(F+1) by 90
 constexpr int GEN_ECAP = 220;                // sum of ncart(l), l <= 9
 constexpr int GEN_FCAP = 165;                // l <= 8
 
@@ -140,7 +139,7 @@ __device__ double hrr_bra(const double* __restrict__ X, int nF, int f, int set_l
 
 __global__ void __launch_bounds__(GEN_LANES)
 eri_generic_kernel(BasisView B, BucketArgs A) {
-  __shared__ double s_u[NRYS_MAX], s_w[NRYS_MAX];
+  __shared__ double s_u[2 * NRYS_MAX], s_w[2 * NRYS_MAX];
   __shared__ double s_V[GEN_VCAP];
   __shared__ int s_ec[GEN_ECAP];
   __shared__ int s_fc[GEN_FCAP];
@@ -193,22 +192,53 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
   const int lij = la + lb + (ip1 ? 1 : 0);
   const int lkl = lc + ld;
+  const int nr = (la + lb + lc + ld + (ip1 ? 1 : 0)) / 2 + 1;  // rys_order
+  const double omega = B.omega;
+  const bool sr = omega < 0;
+  // distance terms of the screening estimate cce = cce0 - l * log(d + 1);
+  // short range widens them to log(d + theta_pair r_guess + 1) and raises
+  // the loop cutoff (src/optimizer.c:301-315, src/cint2e.c:358-380,
+  // src/cint3c2e.c:279-296; r_guess = 8)
+  double dij_term = lij * ld_ij, dkl_term = lkl * ld_kl, Ecut = E;
+  if (sr) {
+    const double w2 = omega * omega;
+    const double dab = sqrt((s0.r[0] - s1.r[0]) * (s0.r[0] - s1.r[0]) +
+                            (s0.r[1] - s1.r[1]) * (s0.r[1] - s1.r[1]) +
+                            (s0.r[2] - s1.r[2]) * (s0.r[2] - s1.r[2]));
+    const double th_ij = w2 / (w2 + B.env[s0.pexp + s0.nprim - 1] + B.env[s1.pexp + s1.nprim - 1]);
+    dij_term = lij > 0 ? lij * log(dab + th_ij * 8.0 + 1.0) : 0.0;
+    if (nr > 1 && lij > 0) Ecut += lij * log((dab + th_ij * 8.0 + 1.0) / (dab + 1.0));
+    if (three) {
+      const double th_k = w2 / (w2 + B.env[s2.pexp + s2.nprim - 1]);
+      if (nr > 1 && lkl > 0) Ecut += lkl * log(th_k * 8.0 + 1.0);
+    } else {
+      const double dcd = sqrt((s2.r[0] - s3.r[0]) * (s2.r[0] - s3.r[0]) +
+                              (s2.r[1] - s3.r[1]) * (s2.r[1] - s3.r[1]) +
+                              (s2.r[2] - s3.r[2]) * (s2.r[2] - s3.r[2]));
+      const double th_kl = w2 / (w2 + B.env[s2.pexp + s2.nprim - 1] + B.env[s3.pexp + s3.nprim - 1]);
+      dkl_term = lkl > 0 ? lkl * log(dcd + th_kl * 8.0 + 1.0) : 0.0;
+      if (nr > 1 && lkl > 0) Ecut += lkl * log((dcd + th_kl * 8.0 + 1.0) / (dcd + 1.0));
+    }
+  }
 
   // pair emptiness (libcint returns early when a pair has no primitive
   // below the cutoff: src/cint2e.c:96-109)
   int live_ij = 0, live_kl = three ? 1 : 0;
-  for (int p = lane; p < nij; p += GEN_LANES) live_ij |= (pij[p].cce0 - lij * ld_ij) < E;
+  for (int p = lane; p < nij; p += GEN_LANES) live_ij |= (pij[p].cce0 - dij_term) < E;
   if (!three)
-    for (int p = lane; p < nkl; p += GEN_LANES) live_kl |= (pkl[p].cce0 - lkl * ld_kl) < E;
+    for (int p = lane; p < nkl; p += GEN_LANES) live_kl |= (pkl[p].cce0 - dkl_term) < E;
   const int live = __syncthreads_and(__syncthreads_or(live_ij) && __syncthreads_or(live_kl));
 
-  const int nr = (la + lb + lc + ld + (ip1 ? 1 : 0)) / 2 + 1;
+  // short range: erfc = 1 - erf, i.e. the full rule on nr roots minus the
+  // theta-scaled rule on nr more (libcint does this for rys_order <= 3,
+  // src/g2e.c:4455-4476, and a dedicated rule on [sqrt(theta), 1] above;
+  // both integrate the polynomial integrand exactly)
+  const int nrt = sr ? 2 * nr : nr;
   const int VS = (G.E + 1) * (G.F + 1);
   const double cfac = PI3_2_SQRTPI * common_fac_sp(A.la) * common_fac_sp(A.lb) *
                       common_fac_sp(A.lc) * (three ? 1.0 : common_fac_sp(ld));
   const double Ax = s0.r[0], Ay = s0.r[1], Az = s0.r[2];
   const double Cx = s2.r[0], Cy = s2.r[1], Cz = s2.r[2];
-  const double omega = B.omega;
   const double* ek = B.env + s2.pexp;
   const double* ck = B.env + s2.pcoef;
   int any = 0;
@@ -227,16 +257,14 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         kkl = ck[kl];
       } else {
         const PairPrim Q = pkl[kl];
-        cce_kl = Q.cce0 - lkl * ld_kl;
+        cce_kl = Q.cce0 - dkl_term;
         akl = Q.aij;
         qx = Q.px; qy = Q.py; qz = Q.pz;
         kkl = Q.kab;
-        if (cce_kl > E) continue;
+        if (cce_kl > Ecut) continue;
       }
-      const double cce_ij = P.cce0 - lij * ld_ij;
-      if (cce_ij > (three ? E : E - cce_kl)) continue;
-      any = 1;
-      if (cce_ij >= E || cce_kl >= E) continue;  // stored exp factor is 0
+      const double cce_ij = P.cce0 - dij_term;
+      if (cce_ij > (three ? Ecut : Ecut - cce_kl)) continue;
 
       const double aij = P.aij;
       const double a1 = aij * akl;
@@ -246,25 +274,39 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
       double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * kkl;
       double x = a0 * rr;
       double theta = 1.0;
-      if (omega > 0) {
+      if (sr) {
+        // pairs beyond the unwidened cutoff carry no data (centre at 1e18),
+        // and tiny erfc tails are dropped: src/g2e.c:4449-4453
+        if (cce_ij >= E || cce_kl >= E) continue;
         theta = omega * omega / (omega * omega + a0);
-        x *= theta;
-        fac1 *= sqrt(theta);
-      }
-      if (lane < nr) {
-        double u, w;
-        rys_root_one(nr, lane, x, u, w);
+        const double cutoff = (three ? Ecut : Ecut - cce_kl) - cce_ij;
+        if (theta * x > cutoff || theta * x > 40.0) continue;  // EXPCUTOFF_SR
+        any = 1;
+      } else {
+        any = 1;
+        if (cce_ij >= E || cce_kl >= E) continue;  // stored exp factor is 0
         if (omega > 0) {
+          theta = omega * omega / (omega * omega + a0);
+          x *= theta;
+          fac1 *= sqrt(theta);
+        }
+      }
+      if (lane < nrt) {
+        double u, w;
+        const bool lr_part = sr && lane >= nr;
+        rys_root_one(nr, lr_part ? lane - nr : lane, lr_part ? x * theta : x, u, w);
+        if (omega > 0 || lr_part) {
           const double ut = u * theta;
           u = ut / (u + 1.0 - ut);
         }
+        if (lr_part) w *= -sqrt(theta);
         s_u[lane] = u;
         s_w[lane] = w * fac1;
       }
       __syncthreads();
-      if (lane < 3 * nr) {
-        const int d = lane / nr;
-        const int r = lane - d * nr;
+      if (lane < 3 * nrt) {
+        const int d = lane / nrt;
+        const int r = lane - d * nrt;
         const double u2 = a0 * s_u[r];
         const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
         const double tmp5 = u2 * tmp4;
@@ -277,7 +319,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         const double qc = d == 0 ? qx - Cx : (d == 1 ? qy - Cy : qz - Cz);
         const double c00 = pa - tmp1 * akl * pq;
         const double c0p = qc + tmp1 * aij * pq;
-        double* V = s_V + (d * nr + r) * VS;
+        double* V = s_V + (d * nrt + r) * VS;
         const int F1 = G.F + 1;
         V[0] = d == 2 ? s_w[r] : 1.0;
         for (int e = 0; e < G.E; ++e) {
@@ -304,9 +346,9 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         const int fx = fc & 15, fy = (fc >> 4) & 15, fz = (fc >> 8) & 15;
         const int F1 = G.F + 1;
         double g = 0;
-        for (int r = 0; r < nr; ++r) {
-          g += s_V[(0 * nr + r) * VS + ex * F1 + fx] * s_V[(1 * nr + r) * VS + ey * F1 + fy] *
-               s_V[(2 * nr + r) * VS + ez * F1 + fz];
+        for (int r = 0; r < nrt; ++r) {
+          g += s_V[(0 * nrt + r) * VS + ex * F1 + fx] * s_V[(1 * nrt + r) * VS + ey * F1 + fy] *
+               s_V[(2 * nrt + r) * VS + ez * F1 + fz];
         }
         if (le >= G.p_lo && le <= G.p_hi) {
           const int e0 = cart_sum(G.p_lo, le - 1) + cart_idx(le, ex, ez);

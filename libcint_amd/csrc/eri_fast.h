@@ -866,11 +866,12 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
 #define MCINT_IP1_GROUP11(X) X(1, 1, 2, 1) X(1, 0, 2, 2) X(1, 2, 1, 0) X(1, 0, 1, 0)
 #define MCINT_IP1_GROUP12(X) X(1, 2, 1, 1) X(1, 2, 2, 0) X(2, 0, 1, 1) X(1, 0, 1, 1)
 #define MCINT_IP1_GROUP13(X) X(2, 1, 1, 1) X(2, 0, 2, 1) X(2, 1, 1, 0) X(1, 0, 2, 0)
+#define MCINT_IP1_GROUP14(X) X(2, 2, 2, 2)
 #define MCINT_IP1_CLASSES(X)                                                        \
   MCINT_IP1_GROUP0(X) MCINT_IP1_GROUP1(X) MCINT_IP1_GROUP2(X) MCINT_IP1_GROUP3(X)   \
   MCINT_IP1_GROUP4(X) MCINT_IP1_GROUP5(X) MCINT_IP1_GROUP6(X) MCINT_IP1_GROUP7(X)   \
   MCINT_IP1_GROUP8(X) MCINT_IP1_GROUP9(X) MCINT_IP1_GROUP10(X) MCINT_IP1_GROUP11(X) \
-  MCINT_IP1_GROUP12(X) MCINT_IP1_GROUP13(X)
+  MCINT_IP1_GROUP12(X) MCINT_IP1_GROUP13(X) MCINT_IP1_GROUP14(X)
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);

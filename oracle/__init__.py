@@ -47,6 +47,7 @@ def lib():
         L.orc_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.orc_rys_roots_exact.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
         L.orc_rys_roots_table.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
+        L.orc_sr_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_set_root_mode.argtypes = [ctypes.c_int]
         L.orc_c2s_matrix.argtypes = [ctypes.c_int, dp]
         L.orc_gto_norm.restype = ctypes.c_double
@@ -78,6 +79,14 @@ def rys_roots(n: int, x: float, exact: bool = True):
     w = np.zeros(n)
     f = lib().orc_rys_roots_exact if exact else lib().orc_rys_roots_table
     f(n, x, _dp(u), _dp(w))
+    return u, w
+
+
+def sr_rys_roots(n: int, x: float, lower: float):
+    """Short-range rule (weight exp(-x t^2) on [lower, 1]): (u, w)."""
+    u = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_sr_rys_roots(n, x, lower, _dp(u), _dp(w))
     return u, w
 
 

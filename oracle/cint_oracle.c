@@ -63,6 +63,7 @@
 #define OC_EXPCUTOFF 60.0     /* src/cint_config.h.in:27 */
 #define OC_MIN_EXPCUTOFF 40.0 /* src/cint_config.h.in:30 */
 #define OC_MAXROOTS 16
+#define OC_EXPCUTOFF_SR 40.0  /* src/rys_roots.h:46 */
 #define OC_PI 3.1415926535897932384626433832795028
 #define OC_SQRTPI 1.7724538509055160272981674833411451
 
@@ -165,10 +166,127 @@ static void ql_first_row(int n, __float128 *d, __float128 *e, __float128 *z)
         }
 }
 
+/* Gauss rule from the moments mu[k] = int t^2k W(t) dt, k < 2n:
+ * Chebyshev algorithm -> Jacobi matrix -> QL; nodes s = t^2 ascending. */
+static void gauss_from_moments_q(int n, const __float128 *mu, __float128 *s_out, __float128 *w_out);
+
 static void rys_exact_q(int n, double x, __float128 *s_out, __float128 *w_out)
 {
         __float128 mu[2 * OC_MAXROOTS];
         boys_q(2 * n - 1, (__float128)x, mu);
+        gauss_from_moments_q(n, mu, s_out, w_out);
+}
+
+/* Short-range rule: the n-point Gauss rule of exp(-x t^2) on [lower, 1]
+ * in the variable s = t^2 (the rule CINTsr_rys_roots returns,
+ * src/rys_roots.c:149-267; its moments are fmt_erfc_like's,
+ * src/fmt.c:340-379).  Ordinary moments are hopelessly ill-conditioned for
+ * narrow [lower, 1], so the rule is built by the discretised Stieltjes
+ * procedure: the weight is sampled by 16-point Gauss-Legendre on
+ * subintervals across which exp(-x t^2) changes by at most e^2 (up to where
+ * it has fallen by e^-100), the three-term recurrence of that discrete
+ * measure is generated in long double, and its Jacobi matrix is
+ * diagonalised like the full-range rule's. */
+static void gauss_legendre16(long double *xg, long double *wg)
+{
+        const int m = 16;
+        for (int i = 0; i < m; i++) {
+                long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (m + 0.5L));
+                for (int it = 0; it < 100; it++) {
+                        long double p0 = 1, p1 = z;
+                        for (int k = 2; k <= m; k++) {
+                                long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                                p0 = p1;
+                                p1 = p2;
+                        }
+                        long double dp = m * (z * p1 - p0) / (z * z - 1);
+                        long double dz = p1 / dp;
+                        z -= dz;
+                        if (fabsl(dz) < 1e-19L) {
+                                xg[i] = z;
+                                wg[i] = 2 / ((1 - z * z) * dp * dp);
+                                break;
+                        }
+                }
+        }
+}
+
+#define SR_MAX_POINTS (16 * 64)
+
+static void rys_sr_rule(int n, double x, double lower, double *u, double *w)
+{
+        long double xg[16], wg[16];
+        gauss_legendre16(xg, wg);
+        long double lo2 = (long double)lower * lower;
+        long double s_end = 1;
+        if (x > 0 && lo2 + 100.0L / x < 1) s_end = lo2 + 100.0L / x;
+        int K = (int)ceill((s_end - lo2) * x / 2);
+        if (K < 4) K = 4;
+        if (K > SR_MAX_POINTS / 16) K = SR_MAX_POINTS / 16;
+        static __thread long double ps[SR_MAX_POINTS], pw[SR_MAX_POINTS];
+        int M = 0;
+        for (int k = 0; k < K; k++) {
+                long double t0 = sqrtl(lo2 + (s_end - lo2) * k / K);
+                long double t1 = sqrtl(lo2 + (s_end - lo2) * (k + 1) / K);
+                long double mid = (t0 + t1) / 2, half = (t1 - t0) / 2;
+                for (int g = 0; g < 16; g++) {
+                        long double t = mid + half * xg[g];
+                        ps[M] = t * t;
+                        pw[M] = half * wg[g] * expl(-(long double)x * t * t);
+                        M++;
+                }
+        }
+        /* Stieltjes: p_{k+1} = (s - a_k) p_k - b_k p_{k-1} on the discrete measure */
+        static __thread long double p0[SR_MAX_POINTS], p1[SR_MAX_POINTS];
+        long double a[OC_MAXROOTS], b[OC_MAXROOTS];
+        long double nrm_prev = 0, mu0 = 0;
+        for (int m = 0; m < M; m++) {
+                p0[m] = 0;
+                p1[m] = 1;
+                mu0 += pw[m];
+        }
+        for (int k = 0; k < n; k++) {
+                long double nrm = 0, sa = 0;
+                for (int m = 0; m < M; m++) {
+                        long double q = pw[m] * p1[m] * p1[m];
+                        nrm += q;
+                        sa += q * ps[m];
+                }
+                a[k] = sa / nrm;
+                b[k] = k ? nrm / nrm_prev : nrm;
+                nrm_prev = nrm;
+                for (int m = 0; m < M; m++) {
+                        long double pn = (ps[m] - a[k]) * p1[m] - (k ? b[k] * p0[m] : 0);
+                        p0[m] = p1[m];
+                        p1[m] = pn;
+                }
+        }
+        __float128 d[OC_MAXROOTS], e[OC_MAXROOTS], z[OC_MAXROOTS];
+        for (int i = 0; i < n; i++) {
+                d[i] = a[i];
+                e[i] = (i + 1 < n) ? sqrtq((__float128)b[i + 1]) : 0;
+        }
+        ql_first_row(n, d, e, z);
+        int idx[OC_MAXROOTS];
+        for (int i = 0; i < n; i++) idx[i] = i;
+        for (int i = 1; i < n; i++)
+                for (int j = i; j > 0 && d[idx[j]] < d[idx[j - 1]]; j--) {
+                        int t = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = t;
+                }
+        for (int i = 0; i < n; i++) {
+                __float128 sv = d[idx[i]];
+                u[i] = (double)(sv / (1 - sv));
+                w[i] = (double)((__float128)mu0 * z[idx[i]] * z[idx[i]]);
+        }
+}
+
+void orc_sr_rys_roots(int n, double x, double lower, double *u, double *w)
+{
+        rys_sr_rule(n, x, lower, u, w);
+}
+
+static void gauss_from_moments_q(int n, const __float128 *mu, __float128 *s_out, __float128 *w_out)
+{
         __float128 sig_prev[2 * OC_MAXROOTS], sig[2 * OC_MAXROOTS], sig_new[2 * OC_MAXROOTS];
         __float128 alpha[OC_MAXROOTS], beta[OC_MAXROOTS];
         for (int k = 0; k < 2 * n; k++) {
@@ -323,7 +441,8 @@ typedef struct {
         int lc[4];     /* ceilings (+1 on i for the ip1 derivative) */
         int nf[4];     /* Cartesian components per shell */
         int nfall;
-        int nroots;
+        int nroots;    /* roots per primitive (2 * rys_order for doubled SR) */
+        int rys_order;
         int ibase, kbase;
         int dli, dlj, dlk, dll;
         int gsi, gsk, gsl, gsj; /* g strides (root fastest) */
@@ -402,7 +521,10 @@ static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas
         if (env[PTR_EXPCUTOFF] == 0) e->expcutoff = OC_EXPCUTOFF;
         else e->expcutoff = fmax(OC_MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]) + 1;
         e->omega = env[PTR_RANGE_OMEGA];
-        e->nroots = (e->lc[0] + e->lc[1] + e->lc[2] + e->lc[3]) / 2 + 1;
+        e->rys_order = (e->lc[0] + e->lc[1] + e->lc[2] + e->lc[3]) / 2 + 1;
+        /* SR with rys_order <= 3 runs full - long-range on doubled roots,
+         * src/g2e.c:73-79 */
+        e->nroots = e->rys_order * ((e->omega < 0 && e->rys_order <= 3) ? 2 : 1);
         oc_strides(e);
 }
 
@@ -431,7 +553,8 @@ static void oc_init_3c2e(OcEnv *e, const int *shls, const int *atm, const int *b
         if (env[PTR_EXPCUTOFF] == 0) e->expcutoff = OC_EXPCUTOFF;
         else e->expcutoff = fmax(OC_MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]);
         e->omega = env[PTR_RANGE_OMEGA];
-        e->nroots = (e->lc[0] + e->lc[1] + e->lc[3]) / 2 + 1;
+        e->rys_order = (e->lc[0] + e->lc[1] + e->lc[3]) / 2 + 1;
+        e->nroots = e->rys_order * ((e->omega < 0 && e->rys_order <= 3) ? 2 : 1);
         oc_strides(e);
 }
 
@@ -545,7 +668,7 @@ static void oc_hrr(double *g, const OcEnv *e)
 /* Per primitive quartet: roots, Rys coefficients, VRR, HRR.
  * src/g2e.c:4425-4545.  Returns 0 when nothing is computed. */
 static int oc_g0_2e(double *g, const OcEnv *e, double aij, double akl,
-                    const double *rij, const double *rkl, double fac)
+                    const double *rij, const double *rkl, double fac, double cutoff)
 {
         int nr = e->nroots;
         double u[OC_MAXROOTS], w[OC_MAXROOTS];
@@ -567,7 +690,22 @@ static int oc_g0_2e(double *g, const OcEnv *e, double aij, double akl,
                         u[r] = ut / (u[r] + 1. - ut);
                 }
         } else {
-                return -1; /* short-range roots are not restated */
+                /* short-range erfc(|omega| r)/r, src/g2e.c:4447-4476 */
+                double theta = e->omega * e->omega / (e->omega * e->omega + a0);
+                if (theta * x > cutoff || theta * x > OC_EXPCUTOFF_SR) return 0;
+                int ro = e->rys_order;
+                if (ro == nr) {
+                        orc_sr_rys_roots(nr, x, sqrt(theta), u, w);
+                } else {
+                        rys_roots(ro, x, u, w);
+                        rys_roots(ro, theta * x, u + ro, w + ro);
+                        double mst = -sqrt(theta);
+                        for (int r = ro; r < nr; r++) {
+                                double ut = u[r] * theta;
+                                u[r] = ut / (u[r] + 1. - ut);
+                                w[r] *= mst;
+                        }
+                }
         }
         double *gz = g + 2 * e->gsize;
         double b00[OC_MAXROOTS], b10[OC_MAXROOTS], b01[OC_MAXROOTS];
@@ -648,19 +786,34 @@ static void oc_gout_ip1(double *gout, const double *g, const double *f, const in
         }
 }
 
+static double oc_dist2(const double *a, const double *b)
+{
+        return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+               (a[2] - b[2]) * (a[2] - b[2]);
+}
+
 /* pair data of one shell pair, src/optimizer.c:288-342.  Writes per
  * primitive pair (ip fastest): cce, eij, rij.  Returns 1 when every pair
  * is screened out. */
 static int oc_pairdata(double *cce, double *eij, double *rij,
                        const double *ai, const double *aj, const double *ci, const double *cj,
                        int npi, int npj, int nci, int ncj,
-                       const double *ri, const double *rj, int lij, double expcutoff)
+                       const double *ri, const double *rj, int lij, double expcutoff,
+                       double omega)
 {
         double rr = 0;
         for (int d = 0; d < 3; d++) rr += (ri[d] - rj[d]) * (ri[d] - rj[d]);
         double a_last = ai[npi - 1] + aj[npj - 1];
         double log_rr = 1.7 - 1.5 * log(a_last);
-        if (lij > 0) log_rr += lij * log(sqrt(rr) + 1.);
+        if (lij > 0) {
+                if (omega < 0) {
+                        /* short range: distance term widened by theta * r_guess */
+                        double theta = omega * omega / (omega * omega + a_last);
+                        log_rr += lij * log(sqrt(rr) + theta * 8. + 1.);
+                } else {
+                        log_rr += lij * log(sqrt(rr) + 1.);
+                }
+        }
         int empty = 1;
         for (int jp = 0; jp < npj; jp++) {
                 double mcj = 0;
@@ -793,7 +946,8 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
         oc_index_xyz(&e, idx);
         double expcutoff = e.expcutoff;
         int empty_ij = oc_pairdata(cce_ij, eij, rij, ex[0], ex[1], cf[0], cf[1], nprim[0], nprim[1],
-                                   nctr[0], nctr[1], e.r[0], e.r[1], e.lc[0] + e.lc[1], expcutoff);
+                                   nctr[0], nctr[1], e.r[0], e.r[1], e.lc[0] + e.lc[1], expcutoff,
+                                   e.omega);
         int empty_kl = 0;
         if (three_center) {
                 /* single centre ket: no pair screening (src/cint3c2e.c:160-211) */
@@ -805,7 +959,32 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
         } else {
                 empty_kl = oc_pairdata(cce_kl, ekl, rkl, ex[2], ex[3], cf[2], cf[3], nprim[2],
                                        nprim[3], nctr[2], nctr[3], e.r[2], e.r[3],
-                                       e.lc[2] + e.lc[3], expcutoff);
+                                       e.lc[2] + e.lc[3], expcutoff, e.omega);
+        }
+        /* short range: wider cutoff for the polynomial prefactors
+         * (ADJUST_CUTOFF, src/cint2e.c:358-380, src/cint3c2e.c:279-296) */
+        if (e.omega < 0 && e.rys_order > 1) {
+                double w2 = e.omega * e.omega;
+                int lij = e.lc[0] + e.lc[1];
+                if (lij > 0) {
+                        double d = sqrt(oc_dist2(e.r[0], e.r[1]));
+                        double th = w2 / (w2 + ex[0][nprim[0] - 1] + ex[1][nprim[1] - 1]);
+                        expcutoff += lij * log((d + th * 8. + 1.) / (d + 1.));
+                }
+                if (three_center) {
+                        int lk = e.lc[3];
+                        if (lk > 0) {
+                                double th = w2 / (w2 + ex[2][nprim[2] - 1]);
+                                expcutoff += lk * log(th * 8. + 1.);
+                        }
+                } else {
+                        int lkl = e.lc[2] + e.lc[3];
+                        if (lkl > 0) {
+                                double d = sqrt(oc_dist2(e.r[2], e.r[3]));
+                                double th = w2 / (w2 + ex[2][nprim[2] - 1] + ex[3][nprim[3] - 1]);
+                                expcutoff += lkl * log((d + th * 8. + 1.) / (d + 1.));
+                        }
+                }
         }
         if (!empty_ij && !empty_kl) {
                 for (int lp = 0; lp < nprim[3]; lp++)
@@ -820,7 +999,8 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
                                 if (cce_ij[nij] > eijcut) continue;
                                 double aij = ex[0][ip] + ex[1][jp];
                                 double fac = e.common_factor * eij[nij] * ekl[nkl];
-                                int st = oc_g0_2e(g, &e, aij, akl, rij + 3 * nij, rkl + 3 * nkl, fac);
+                                int st = oc_g0_2e(g, &e, aij, akl, rij + 3 * nij, rkl + 3 * nkl, fac,
+                                                  eijcut - cce_ij[nij]);
                                 if (st < 0) { not0 = -1; goto done; }
                                 if (!st) continue;
                                 not0 = 1;

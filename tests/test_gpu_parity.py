@@ -215,3 +215,61 @@ def test_schwarz_bounds(mc, orc):
             assert qb[j, i] == qb[i, j]
     finally:
         orc.set_root_mode(orc.ROOTS_EXACT)
+
+
+def _with_omega(basis_arrays, omega):
+    atm, bas, env = basis_arrays
+    env = np.array(env, dtype=np.float64, copy=True)
+    env[8] = omega  # PTR_RANGE_OMEGA
+    return atm, bas, env
+
+
+@pytest.mark.parametrize("omega", [0.5, -0.5, -0.2])
+def test_range_separated_fixture(mc, orc, fixture_basis, omega):
+    """Long-range (omega > 0) and short-range (omega < 0) int2e_sph on the
+    libcint fixture quartets against the oracle, which uses libcint's own
+    short-range rule on [sqrt(theta), 1] for rys_order > 3 (the device takes
+    full minus long range on doubled roots for every order)."""
+    _say("fixture int2e_sph omega", omega)
+    q = libcint_quartets()
+    arr = _with_omega(fixture_basis, omega)
+    got, off, not0 = mc.batch("int2e_sph", q, mc.Basis(*arr))
+    ref, off_r, not0_r = orc.batch("int2e_sph", q, orc.Basis(*arr))
+    _cmp(got, ref, f"int2e_sph omega={omega}")
+    np.testing.assert_array_equal(not0, not0_r)
+
+
+def test_erf_plus_erfc_is_coulomb(mc, fixture_basis):
+    """testsuite/test_cint.py:258-302 on the device: LR + SR = full."""
+    q = libcint_quartets()
+    full, _, _ = mc.batch("int2e_sph", q, mc.Basis(*fixture_basis))
+    for omega in (0.2, 0.5, 0.8):
+        _say("erf + erfc omega", omega)
+        lr, _, _ = mc.batch("int2e_sph", q, mc.Basis(*_with_omega(fixture_basis, omega)))
+        sr, _, _ = mc.batch("int2e_sph", q, mc.Basis(*_with_omega(fixture_basis, -omega)))
+        assert np.round(np.abs(lr + sr - full).max(), 9) == 0
+
+
+@pytest.mark.parametrize("omega", [-0.3, 0.3])
+def test_c4_3c2e_range_separated_sample(mc, orc, omega):
+    """The 40-atom C/H cluster's (ij|P) with the erfc (omega = -0.3) and erf
+    (omega = 0.3) operators against the oracle; aux shells up to g, so the
+    short-range rys_order reaches 6."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c4")
+    rng = np.random.default_rng(7)
+    n = 400
+    q = np.zeros((n, 4), np.int32)
+    q[:, 0] = rng.integers(0, m.aux_start, n)
+    q[:, 1] = rng.integers(0, m.aux_start, n)
+    q[:, 2] = rng.integers(m.aux_start, m.nbas, n)
+    arr = _with_omega((m.atm, m.bas, m.env), omega)
+    _say("c4 int3c2e omega", omega)
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        got, _, nz = mc.batch("int3c2e_sph", q, mc.Basis(*arr))
+        ref, _, nz_r = orc.batch("int3c2e_sph", q, orc.Basis(*arr))
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)
+    _cmp(got, ref, f"c4 int3c2e_sph omega={omega}")
+    np.testing.assert_array_equal(nz, nz_r)

@@ -226,3 +226,43 @@ def test_ip1_finite_difference(orc):
             vm, _ = orc.intor("int2e_cart", (i, j, k, l), bm)
             fd = -(vp - vm) / (2 * h)
             np.testing.assert_allclose(g[..., d], fd, rtol=0, atol=2e-7 * max(1.0, np.abs(fd).max()))
+
+
+def test_sr_rys_moments(orc):
+    """Short-range rule (src/rys_roots.c:149-267): the n-point Gauss rule of
+    exp(-x t^2) on [lower, 1], checked by its moments against mpmath's
+    quadrature of the same integral, k < 2n."""
+    mp.mp.dps = 30
+    for n in (1, 2, 3, 4, 5, 6, 7):
+        for x in (0.0, 0.3, 2.5, 11.0, 37.0):
+            for lower in (0.05, 0.4, 0.8, 0.97):
+                u, w = orc.sr_rys_roots(n, x, lower)
+                s = u / (1 + u)
+                assert np.all(w > 0) and np.all(s >= lower * lower * (1 - 1e-12))
+                for k in range(2 * n):
+                    ref = float(mp.quad(lambda t: t ** (2 * k) * mp.e ** (-x * t * t), [lower, 1]))
+                    got = float(np.sum(w * s ** k))
+                    assert abs(got - ref) <= 1e-13 * ref, (n, x, lower, k, got, ref)
+
+
+@pytest.mark.parametrize("omega", [0.2, 0.5, 0.8])
+def test_erf_plus_erfc_is_coulomb(orc, fixture_basis, omega):
+    """testsuite/test_cint.py:258-302: erf(w r)/r + erfc(w r)/r = 1/r for
+    every quartet (i<=j, k<=l) of the fixture to 9 places, with the oracle's
+    short-range path (doubled roots for rys_order <= 3, the [sqrt(theta), 1]
+    rule above) and long-range path; also for (ij|k) 3-centre integrals."""
+    atm, bas, env = fixture_basis
+    q = libcint_quartets()
+    q3 = np.array([(i, j, k, 0) for k in range(8) for j in range(4) for i in range(4)], np.int32)
+    res = {}
+    for o in (0.0, omega, -omega):
+        e2 = np.array(env, copy=True)
+        e2[8] = o
+        B = orc.Basis(atm, bas, e2)
+        res[o] = (orc.batch("int2e_sph", q, B, nthreads=8)[0],
+                  orc.batch("int3c2e_sph", q3, B, nthreads=8)[0])
+    for t in range(2):
+        d = res[omega][t] + res[-omega][t] - res[0.0][t]
+        assert np.round(np.abs(d).max(), 9) == 0, (omega, t, np.abs(d).max())
+        # the short-range part is a genuine fraction of the full integral
+        assert np.abs(res[-omega][t]).max() > 1e-3 * np.abs(res[0.0][t]).max()
