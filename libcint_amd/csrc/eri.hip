@@ -140,7 +140,7 @@ __device__ double hrr_bra(const double* __restrict__ X, int nF, int f, int set_l
 __global__ void __launch_bounds__(GEN_LANES)
 eri_generic_kernel(BasisView B, BucketArgs A) {
   __shared__ double s_u[2 * NRYS_MAX], s_w[2 * NRYS_MAX];
-  __shared__ double s_V[GEN_VCAP];
+  extern __shared__ double s_V[];  // 3 * roots * (E+1)(F+1), sized at launch
   __shared__ int s_ec[GEN_ECAP];
   __shared__ int s_fc[GEN_FCAP];
 
@@ -612,7 +612,13 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
   const int64_t n = A.t1 - A.t0;
   if (n <= 0) return hipSuccess;
   // grid.x is limited to 2^31-1 workgroups; callers split larger buckets
-  hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)n), dim3(GEN_LANES), 0, stream, B, A);
+  const GenLayout G = gen_layout(A.kind, A.sph, A.la, A.lb, A.lc, A.ld);
+  const int nr = (A.la + A.lb + A.lc + (A.kind == KIND_3C2E ? 0 : A.ld) +
+                  (A.kind == KIND_2E_IP1 ? 1 : 0)) / 2 + 1;
+  const int nrt = B.omega < 0 ? 2 * nr : nr;
+  const size_t lds = sizeof(double) * 3 * nrt * (G.E + 1) * (G.F + 1);
+  if (lds > sizeof(double) * GEN_VCAP) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)n), dim3(GEN_LANES), lds, stream, B, A);
   return hipGetLastError();
 }
 

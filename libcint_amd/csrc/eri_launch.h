@@ -9,9 +9,9 @@ namespace mcint {
 //     the i-derivative; XM: the unweighted lowered set of the derivative;
 // H1: bra transfer (Cartesian); H2: bra Cartesian->spherical;
 // H3: ket transfer (Cartesian, 4-centre only).
-// LDS capacity (doubles) of the generic kernel's recurrence arrays:
+// LDS bound (doubles) of the generic kernel's recurrence arrays:
 // 3 directions x roots (doubled for short range) x (E+1)(F+1)
-constexpr int GEN_VCAP = 3 * NRYS_MAX * 90;
+constexpr int GEN_VCAP = 3 * 2 * NRYS_MAX * 90;
 
 struct GenLayout {
   int e_lo, E, F;         // e shells e_lo..E, f shells lc..F
