@@ -235,6 +235,15 @@ void mcint_jk_plan_destroy(mcint_jk_plan *p);
  * all).  q is a host array of nbas*nbas doubles. */
 int mcint_schwarz(mcint_basis *b, int nshell, double *q);
 
+/* Tuning hook for plan construction (no libcint counterpart; used by the
+ * benchmark's autotuner).  MCINT_TUNE_PRIMS_PER_LANE: the team of lanes
+ * that shares one contracted quartet is the smallest power of two (<= 64)
+ * giving at most this many primitive quartets per lane (default 32).
+ * Applies to plans created afterwards; returns the previous value, or a
+ * negative status. */
+#define MCINT_TUNE_PRIMS_PER_LANE 0
+int mcint_tune(int key, int value);
+
 const char *mcint_last_error(void);
 
 #ifdef __cplusplus

@@ -103,6 +103,8 @@ def lib():
     L.mcint_jk_plan_destroy.argtypes = [vp]
     L.mcint_schwarz.restype = ctypes.c_int
     L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
+    L.mcint_tune.restype = ctypes.c_int
+    L.mcint_tune.argtypes = [ctypes.c_int, ctypes.c_int]
     L.mcint_last_error.restype = ctypes.c_char_p
     L.CINTrys_roots.restype = ctypes.c_int
     L.CINTrys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
@@ -112,6 +114,17 @@ def lib():
     L.CINTgto_norm.argtypes = [ctypes.c_int, ctypes.c_double]
     _lib = L
     return L
+
+
+TUNE_PRIMS_PER_LANE = 0
+
+
+def tune(key: int, value: int) -> int:
+    """Set a plan-construction tuning value (mcint_tune); returns the old one."""
+    rc = lib().mcint_tune(key, value)
+    if rc < 0:
+        raise McintError(f"mcint_tune failed ({rc}): {last_error()}")
+    return rc
 
 
 def last_error() -> str:

@@ -205,9 +205,13 @@ constexpr int64_t GEN_SCRATCH_DOUBLES = 64ll << 20;
 // copies of the gradient accumulators (spreads the digest's atomics)
 constexpr int GRAD_SLOTS = 64;
 
+// Tuning state (mcint_tune): primitive quartets per lane that the team
+// size aims at.
+int g_prims_per_lane = 32;
+
 int pick_team(int64_t nprim_quartets) {
   int t = 1;
-  while (t < fast_team_lanes_max() && (int64_t)t * 8 < nprim_quartets) t *= 2;
+  while (t < fast_team_lanes_max() && (int64_t)t * g_prims_per_lane < nprim_quartets) t *= 2;
   return t;
 }
 
@@ -254,6 +258,20 @@ static void plan_free(mcint_plan* p) {
 }
 
 extern "C" void mcint_plan_destroy(mcint_plan* p) { plan_free(p); }
+
+extern "C" int mcint_tune(int key, int value) {
+  if (key == MCINT_TUNE_PRIMS_PER_LANE) {
+    if (value < 1 || value > 4096) {
+      set_err("mcint_tune: primitive quartets per lane must be in [1, 4096]");
+      return MCINT_ERR_ARG;
+    }
+    const int old = g_prims_per_lane;
+    g_prims_per_lane = value;
+    return old;
+  }
+  set_err("mcint_tune: unknown key");
+  return MCINT_ERR_ARG;
+}
 
 static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
                             size_t nq, const int64_t* out_offsets, const uint8_t* fp32_flags,

@@ -143,7 +143,8 @@ __device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, 
 // primitive quartet that passed libcint's two screening tests.
 template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
-                                           R (&acc)[K1 - K0], int& any, int& live) {
+                                           const double* __restrict__ rtab, R (&acc)[K1 - K0],
+                                           int& any, int& live) {
   using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
   const VShell& sa = Bv.vsh[tk.sh[0]];
@@ -203,19 +204,19 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
 
     const double aij = P.aij, akl = Q.aij;
     const double a1 = aij * akl;
-    const double a0 = a1 / (aij + akl);
+    const double a0 = a1 * rcp_pos(aij + akl);
     const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
     const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
-    const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * Q.kab;
+    const double fac1 = sqrt(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
     const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
     double u[NR], w[NR];
-    rys_roots<NR>(a0 * rr, u, w);
+    rys_roots<NR>(a0 * rr, u, w, rtab);
     const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
     const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
     sfor<NR>([&](auto RI) {
       constexpr int r = decltype(RI)::value;
       const double u2 = a0 * u[r];
-      const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+      const double tmp4 = 0.5 * rcp_pos(u2 * (aij + akl) + a1);
       const double b00 = u2 * tmp4;
       const double b10 = b00 + tmp4 * akl;
       const double b01 = b00 + tmp4 * aij;
@@ -267,6 +268,8 @@ template <int LA, int LB, int LC, int LD, typename R = double>
 __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int NE = C::NE, NF = C::NF;
+  __shared__ double s_rys[RysLds<C::NR>::N];
+  const double* rtab = rys_stage_table<C::NR>(s_rys);
   const int T = A.team;
   const int tid = threadIdx.x;
   const int tl = tid & (T - 1);
@@ -281,7 +284,7 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
   R acc[NE * NF];
   int any, live;
-  accumulate<LA, LB, LC, LD, 0, NE * NF, R>(Bv, tk, T, tl, acc, any, live);
+  accumulate<LA, LB, LC, LD, 0, NE * NF, R>(Bv, tk, T, tl, rtab, acc, any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -413,6 +416,8 @@ template <int LA, int LB, int LC, int LD, int CH, int DER = 0>
 __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs A) {
   using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
+  __shared__ double s_rys[RysLds<G::C::NR>::N];
+  const double* rtab = rys_stage_table<G::C::NR>(s_rys);
   const int T = A.team;
   const int tid = threadIdx.x;
   const int tl = tid & (T - 1);
@@ -421,7 +426,7 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   const Task tk = A.tasks[t];
   double acc[K1 - K0];
   int any, live;
-  accumulate<LA, LB, LC, LD, K0, K1, double, DER>(Bv, tk, T, tl, acc, any, live);
+  accumulate<LA, LB, LC, LD, K0, K1, double, DER>(Bv, tk, T, tl, rtab, acc, any, live);
   for (int off = 1; off < T; off <<= 1) {
     sfor<K1 - K0>([&](auto I) {
       constexpr int i = decltype(I)::value;

@@ -10,6 +10,17 @@
 
 namespace mcint {
 
+// 1/d for positive normal d: hardware reciprocal refined by two Newton
+// steps (error below 1 ulp; the IEEE division sequence costs about twice
+// the instructions and handles cases that cannot occur here).
+__device__ __forceinline__ double rcp_pos(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Root/weight number f (0 <= f < 2n: roots first, then weights) of the
 // n-point rule at x < RYS_XMAX(n).
 __device__ __forceinline__ double rys_poly_eval(int n, int f, double x) {
@@ -34,10 +45,34 @@ __device__ __forceinline__ void rys_root_one(int n, int r, double x, double& u, 
   }
 }
 
-// All NR roots (compile-time NR): one Horner chain per function, the
-// coefficients of one interval are contiguous per lane.
+// Root tables small enough to stage in LDS (n <= 4: at most 52.5 KB).
+constexpr int RYS_LDS_NMAX = 4;
 template <int NR>
-__device__ __forceinline__ void rys_roots(double x, double* u, double* w) {
+struct RysLds {
+  static constexpr int N = NR <= RYS_LDS_NMAX ? RYS_XMAX(NR) * RYS_NCOEF * 2 * NR : 1;
+};
+
+// Copies the n = NR table into s (all threads of the block; ends with a
+// barrier) and returns the table to evaluate from: s, or the global copy
+// when the table does not fit.
+template <int NR>
+__device__ __forceinline__ const double* rys_stage_table(double* s) {
+  if constexpr (NR <= RYS_LDS_NMAX) {
+    const double* g = rys_coef + rys_table_offset[NR];
+    for (int i = threadIdx.x; i < RysLds<NR>::N; i += blockDim.x) s[i] = g[i];
+    __syncthreads();
+    return s;
+  } else {
+    return rys_coef + rys_table_offset[NR];
+  }
+}
+
+// All NR roots (compile-time NR): one Horner chain per function, the
+// coefficients of one interval are contiguous per lane.  tab: the n = NR
+// table (rys_coef + rys_table_offset[NR], or its LDS copy).
+template <int NR>
+__device__ __forceinline__ void rys_roots(double x, double* u, double* w,
+                                          const double* __restrict__ tab) {
   if (x >= (double)RYS_XMAX(NR)) {
     const double rsx = 1.0 / sqrt(x);
 #pragma unroll
@@ -49,7 +84,7 @@ __device__ __forceinline__ void rys_roots(double x, double* u, double* w) {
   } else {
     const int it = (int)x;
     const double z = 2.0 * (x - (double)it) - 1.0;
-    const double* c = rys_coef + rys_table_offset[NR] + (size_t)it * RYS_NCOEF * 2 * NR;
+    const double* c = tab + it * (RYS_NCOEF * 2 * NR);
     double acc[2 * NR];
 #pragma unroll
     for (int f = 0; f < 2 * NR; ++f) acc[f] = c[(RYS_NCOEF - 1) * 2 * NR + f];
