@@ -459,6 +459,7 @@ struct EpiCfg {
   static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
   static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
   static constexpr int THREADS = TPB * TE;
+  static_assert(PER <= 6144, "epilogue exceeds its LDS budget");
 };
 
 template <int LA, int LB, int LC, int LD>
@@ -836,8 +837,9 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
   MCINT_FAST_GROUP4(X) MCINT_FAST_GROUP5(X) MCINT_FAST_GROUP6(X) MCINT_FAST_GROUP7(X) \
   MCINT_FAST_GROUP8(X)
 
-// Classes on the chunked path ([e0|f0] block of at most 800 doubles), one
-// translation unit per group (chunk_<g>.hip).
+// Classes on the chunked path ([e0|f0] blocks above the single-kernel
+// budget whose epilogue fits 48 KB of LDS), one translation unit per group
+// (chunk_<g>.hip).
 #define MCINT_CHUNK_GROUP0(X) X(3, 1, 1, 1) X(3, 1, 3, 0)
 #define MCINT_CHUNK_GROUP1(X) X(2, 1, 2, 1) X(2, 2, 2, 0)
 #define MCINT_CHUNK_GROUP2(X) X(2, 2, 1, 1) X(3, 3, 1, 0)
@@ -849,10 +851,19 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
 #define MCINT_CHUNK_GROUP8(X) X(3, 2, 2, 1)
 #define MCINT_CHUNK_GROUP9(X) X(3, 3, 3, 0)
 #define MCINT_CHUNK_GROUP10(X) X(3, 1, 2, 2)
-#define MCINT_CHUNK_CLASSES(X)                                                            \
-  MCINT_CHUNK_GROUP0(X) MCINT_CHUNK_GROUP1(X) MCINT_CHUNK_GROUP2(X) MCINT_CHUNK_GROUP3(X) \
-  MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X) \
-  MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X)
+// larger blocks (961..3404 doubles), f and d shells
+#define MCINT_CHUNK_GROUP11(X) X(3, 3, 3, 2)
+#define MCINT_CHUNK_GROUP12(X) X(3, 3, 2, 2)
+#define MCINT_CHUNK_GROUP13(X) X(3, 2, 3, 2)
+#define MCINT_CHUNK_GROUP14(X) X(3, 3, 3, 1)
+#define MCINT_CHUNK_GROUP15(X) X(3, 2, 2, 2) X(2, 2, 2, 2)
+#define MCINT_CHUNK_GROUP16(X) X(3, 3, 2, 1) X(3, 2, 3, 1)
+#define MCINT_CHUNK_CLASSES(X)                                                              \
+  MCINT_CHUNK_GROUP0(X) MCINT_CHUNK_GROUP1(X) MCINT_CHUNK_GROUP2(X) MCINT_CHUNK_GROUP3(X)   \
+  MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X)   \
+  MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X) MCINT_CHUNK_GROUP11(X) \
+  MCINT_CHUNK_GROUP12(X) MCINT_CHUNK_GROUP13(X) MCINT_CHUNK_GROUP14(X) MCINT_CHUNK_GROUP15(X) \
+  MCINT_CHUNK_GROUP16(X)
 
 // i-derivative (int2e_ip1) classes on the chunked path: shells up to d,
 // canonical LC >= LD (the derivative shell i stays first), one translation
