@@ -80,8 +80,9 @@ def gradient_section(reps: int = 3) -> dict:
     """The derivative and J/K legs on the 20-atom C8H12 cluster (DZ basis
     with d on H): plain FP64 int2e_sph over all unique quartets, the J/K
     build (ERIs + grouped digest, FP64 and with the FP32 tier), and the
-    nuclear-gradient digest over the ip1 list (i, j, k >= l), each timed on
-    device-resident data after one untimed run."""
+    nuclear-gradient digest over the ip1 list (i, j, k >= l), FP64 and with
+    the FP32 tier (Schwarz x max|D|^2 < 1e-7), each timed on device-resident
+    data after one untimed run."""
     import torch
     import libcint_amd as mc
     from libcint_amd import jk
@@ -126,6 +127,11 @@ def gradient_section(reps: int = 3) -> dict:
     gsum = float(g.sum(0).abs().max())
     nq_grad = gp.nquartets
     gp.close()
+    gp32 = jk.GradientPlan(B, m.natm, fp32_threshold=1e-7, dmax=float(np.abs(dm).max()))
+    g32_tasks = gp32.stats()["fp32_tasks"]
+    t_grad32, g32 = timed(lambda: gp32.run(dm))
+    g32_err = float((g32 - g).abs().max())
+    gp32.close()
     db.close()
     return {
         "workload": f"c3: {m.name}, {m.natm} atoms, {m.nbas} shells, {m.nao_sph()} AOs; "
@@ -136,6 +142,9 @@ def gradient_section(reps: int = 3) -> dict:
         "ip1_gradient_quartets": int(nq_grad),
         "ip1_gradient_vs_plain": t_grad / t_plain,
         "gradient_plan_create_s": t_gplan,
+        "ip1_gradient_mixed_ms": t_grad32 * 1e3,
+        "ip1_gradient_mixed_fp32_quartets": int(g32_tasks),
+        "ip1_gradient_mixed_max_abs_error": g32_err,
         "gradient_translational_residual": gsum,
         "jk_fp64_ms": t_jk * 1e3,
         "jk_mixed_ms": t_jk32 * 1e3,

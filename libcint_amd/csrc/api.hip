@@ -369,9 +369,11 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     // range-separated operators run on the generic kernel
     const int mode = b->omega != 0 ? 0 : fast_kernel_mode(kind, sph, la, lb, lc, ld);
     const int team = mode ? pick_team(nprimq) : 0;
-    // FP32 recurrences only where a single-chunk kernel exists; other
-    // flagged quartets stay FP64
-    const uint32_t lowp = (fp32_flags && fp32_flags[q] && mode == 1) ? 1u : 0u;
+    // FP32 recurrences where a single-chunk kernel or a derivative chunk
+    // kernel exists; other flagged quartets stay FP64
+    const uint32_t lowp =
+        (fp32_flags && fp32_flags[q] && (mode == 1 || (mode == 2 && kind == KIND_2E_IP1))) ? 1u
+                                                                                           : 0u;
     int tl = 0;
     while ((1 << tl) < team) ++tl;
     const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |
@@ -457,9 +459,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     bk.team = 1 << ((k >> 20) & 15);
     bk.mode = (k >> 24) & 3;
     bk.fp32 = (k >> 26) & 1;
-    if (bk.fp32) p->stats_fp32 += bk.t1 - bk.t0;
     bk.t0 = cnt[i];
     bk.t1 = cnt[i + 1];
+    if (bk.fp32) p->stats_fp32 += bk.t1 - bk.t0;
     bk.sstride = 0;
     if (bk.mode == 2) {
       const int64_t nef = chunked_block_doubles(kind, bk.la, bk.lb, bk.lc, bk.ld);

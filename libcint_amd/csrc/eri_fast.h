@@ -141,7 +141,8 @@ __device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, 
 // index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
 // is the team's pair-emptiness verdict, 'any' whether this lane saw a
 // primitive quartet that passed libcint's two screening tests.
-template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
+template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0,
+          int RL = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
                                            const double* __restrict__ rtab, R (&acc)[K1 - K0],
                                            int& any, int& live) {
@@ -209,13 +210,11 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
     const double fac1 = sqrt(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
     const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
-    double u[NR], w[NR];
-    rys_roots<NR>(a0 * rr, u, w, rtab);
     const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
     const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
-    sfor<NR>([&](auto RI) {
-      constexpr int r = decltype(RI)::value;
-      const double u2 = a0 * u[r];
+    // one root: coefficients, recurrence, products
+    auto root_body = [&](double ur, double wr) {
+      const double u2 = a0 * ur;
       const double tmp4 = 0.5 * rcp_pos(u2 * (aij + akl) + a1);
       const double b00 = u2 * tmp4;
       const double b10 = b00 + tmp4 * akl;
@@ -226,7 +225,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       R Vx[E + 1][F + 1], Vy[E + 1][F + 1], Vz[E + 1][F + 1];
       vrr2d<E, F, R>(Vx, R(1), R(pax - tmp2 * xpq), R(qcx + tmp3 * xpq), R(b10), R(b01), R(b00));
       vrr2d<E, F, R>(Vy, R(1), R(pay - tmp2 * ypq), R(qcy + tmp3 * ypq), R(b10), R(b01), R(b00));
-      vrr2d<E, F, R>(Vz, R(w[r] * fac1), R(paz - tmp2 * zpq), R(qcz + tmp3 * zpq), R(b10), R(b01),
+      vrr2d<E, F, R>(Vz, R(wr * fac1), R(paz - tmp2 * zpq), R(qcz + tmp3 * zpq), R(b10), R(b01),
                      R(b00));
       sfor<K1 - K0>([&](auto KK) {
         constexpr int k = K0 + decltype(KK)::value;
@@ -239,7 +238,21 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
         else
           acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
       });
-    });
+    };
+    const double x = a0 * rr;
+    if constexpr (RL) {
+      // rolled root loop: one root's state live at a time
+#pragma nounroll
+      for (int r = 0; r < NR; ++r) {
+        double ur, wr;
+        rys_root_sel<NR>(x, r, ur, wr, rtab);
+        root_body(ur, wr);
+      }
+    } else {
+      double u[NR], w[NR];
+      rys_roots<NR>(x, u, w, rtab);
+      sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+    }
   }
 }
 
@@ -264,7 +277,7 @@ __device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& t
   st3 = so[(perm >> 6) & 3];
 }
 
-template <int LA, int LB, int LC, int LD, typename R = double>
+template <int LA, int LB, int LC, int LD, typename R = double, int RL = 0>
 __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int NE = C::NE, NF = C::NF;
@@ -284,7 +297,7 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
   R acc[NE * NF];
   int any, live;
-  accumulate<LA, LB, LC, LD, 0, NE * NF, R>(Bv, tk, T, tl, rtab, acc, any, live);
+  accumulate<LA, LB, LC, LD, 0, NE * NF, R, 0, RL>(Bv, tk, T, tl, rtab, acc, any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -412,7 +425,7 @@ struct ChunkCfg {
   static constexpr int k1(int c) { return (c + 1) * W < NEF ? (c + 1) * W : NEF; }
 };
 
-template <int LA, int LB, int LC, int LD, int CH, int DER = 0>
+template <int LA, int LB, int LC, int LD, int CH, int DER = 0, typename R = double>
 __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs A) {
   using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
@@ -424,9 +437,9 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
   if (t >= A.t1) return;
   const Task tk = A.tasks[t];
-  double acc[K1 - K0];
+  R acc[K1 - K0];
   int any, live;
-  accumulate<LA, LB, LC, LD, K0, K1, double, DER>(Bv, tk, T, tl, rtab, acc, any, live);
+  accumulate<LA, LB, LC, LD, K0, K1, R, DER>(Bv, tk, T, tl, rtab, acc, any, live);
   for (int off = 1; off < T; off <<= 1) {
     sfor<K1 - K0>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -438,7 +451,7 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   double* __restrict__ dst = A.scratch + slot * G::NEF;
   sfor<K1 - K0>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    if ((i & (T - 1)) == tl) dst[K0 + i] = acc[i];
+    if ((i & (T - 1)) == tl) dst[K0 + i] = (double)acc[i];
   });
   if constexpr (CH == 0) {
     if (tl == 0) {
@@ -452,7 +465,10 @@ template <int LA, int LB, int LC, int LD>
 struct EpiCfg {
   using C = FastClass<LA, LB, LC, LD>;
   static constexpr int NEF = C::NE * C::NF;
-  static constexpr int PER = NEF + C::NAo * C::NBo * C::NF;  // doubles of LDS per task
+  // the [e0|f0] block is staged in LDS unless block + bra result exceed
+  // 48 KB; then the bra transfer reads it from the scratch block
+  static constexpr bool XG = NEF + C::NAo * C::NBo * C::NF > 6144;
+  static constexpr int PER = (XG ? 0 : NEF) + C::NAo * C::NBo * C::NF;  // LDS doubles per task
   static constexpr int WORK = C::NF > C::NAo * C::NBo ? C::NF : C::NAo * C::NBo;
   static constexpr int TE = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
   static constexpr int TPB0 = 256 / TE;
@@ -462,77 +478,69 @@ struct EpiCfg {
   static_assert(PER <= 6144, "epilogue exceeds its LDS budget");
 };
 
+// Bra transfer (a,b| = sum_t C(b,t) (A-B)^(b-t) [a+t| and the spherical
+// transform over a, b for the columns j = j0, j0+js, ... of the contracted
+// block X (NE x NF), written to Ys (NAo*NBo x NF).
 template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A) {
+__device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
+                                        const double* __restrict__ X, double* __restrict__ Ys,
+                                        int j0, int js) {
   using C = FastClass<LA, LB, LC, LD>;
-  using P = EpiCfg<LA, LB, LC, LD>;
-  constexpr int NE = C::NE, NF = C::NF, NEF = P::NEF, TE = P::TE;
-  constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
-  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
-  __shared__ double sm[P::TPB * P::PER];
-  const int team = threadIdx.x / TE;
-  const int tl = threadIdx.x % TE;
-  const int64_t t = A.t0 + (int64_t)blockIdx.x * P::TPB + team;
-  const bool valid = t < A.t1;
-  double* X = sm + team * P::PER;
-  double* Ys = X + NEF;
-  int any = 0;
-  Task tk;
-  if (valid) {
-    tk = A.tasks[t];
-    const int64_t slot = t - A.t0;
-    any = A.flags[slot];
-    const double* src = A.scratch + slot * NEF;
-    if (any)
-      for (int k = tl; k < NEF; k += TE) X[k] = src[k];
-  }
-  __syncthreads();
-  if (valid && any) {
-    const VShell& sa = Bv.vsh[tk.sh[0]];
-    const VShell& sb = Bv.vsh[tk.sh[1]];
-    const double ab[3] = {sa.r[0] - sb.r[0], sa.r[1] - sb.r[1], sa.r[2] - sb.r[2]};
-    double abp[3][LB + 1];
-    sfor<3>([&](auto DI) {
-      constexpr int d = decltype(DI)::value;
-      abp[d][0] = 1.0;
-      sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
+  constexpr int NF = C::NF, NA = C::NA, NB = C::NB, NAo = C::NAo, NBo = C::NBo;
+  const VShell& sa = Bv.vsh[tk.sh[0]];
+  const VShell& sb = Bv.vsh[tk.sh[1]];
+  const double ab[3] = {sa.r[0] - sb.r[0], sa.r[1] - sb.r[1], sa.r[2] - sb.r[2]};
+  double abp[3][LB + 1];
+  sfor<3>([&](auto DI) {
+    constexpr int d = decltype(DI)::value;
+    abp[d][0] = 1.0;
+    sfor<LB>([&](auto K) { abp[d][decltype(K)::value + 1] = abp[d][decltype(K)::value] * ab[d]; });
+  });
+  for (int j = j0; j < NF; j += js) {
+    double Yc[NA * NB];
+    sfor<NA * NB>([&](auto AB) {
+      constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
+      constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
+      constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
+      double s = 0.0;
+      sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
+        constexpr int tt = decltype(TT)::value;
+        constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
+        constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
+        constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
+        const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
+        s = fma(cb * pw, X[e * NF + j], s);
+      });
+      Yc[a * NB + b] = s;
     });
-    for (int j = tl; j < NF; j += TE) {
-      double Yc[NA * NB];
+    sfor<NAo * NBo>([&](auto OB) {
+      constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
+      double s = 0.0;
       sfor<NA * NB>([&](auto AB) {
         constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
-        constexpr int ax = cpx(LA, a), ay = cpy(LA, a), az = cpz(LA, a);
-        constexpr int bx = cpx(LB, b), by = cpy(LB, b), bz = cpz(LB, b);
-        double s = 0.0;
-        sfor<(bx + 1) * (by + 1) * (bz + 1)>([&](auto TT) {
-          constexpr int tt = decltype(TT)::value;
-          constexpr int tx = tt / ((by + 1) * (bz + 1)), ty = (tt / (bz + 1)) % (by + 1), tz = tt % (bz + 1);
-          constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
-          constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
-          const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
-          s = fma(cb * pw, X[e * NF + j], s);
-        });
-        Yc[a * NB + b] = s;
+        constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+        if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
       });
-      sfor<NAo * NBo>([&](auto OB) {
-        constexpr int ao = decltype(OB)::value / NBo, bo = decltype(OB)::value % NBo;
-        double s = 0.0;
-        sfor<NA * NB>([&](auto AB) {
-          constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
-          constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
-          if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
-        });
-        Ys[(ao * NBo + bo) * NF + j] = s;
-      });
-    }
+      Ys[(ao * NBo + bo) * NF + j] = s;
+    });
   }
-  __syncthreads();
-  if (!valid) return;
+}
+
+// Ket transfer and spherical c, d of the rows r0, r0+rs, ... of Ys, then
+// the store with the task's output strides; any == 0 writes zeros.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& tk, int perm,
+                                              const double* __restrict__ Ys,
+                                              double* __restrict__ outbase, int any, int r0,
+                                              int rs) {
+  using C = FastClass<LA, LB, LC, LD>;
+  constexpr int NF = C::NF, NC = C::NC, ND = C::ND;
+  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
   int64_t st0, st1, st2, st3;
-  canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
-  double* __restrict__ out = A.out + tk.out;
+  canon_strides(Bv, tk, perm, st0, st1, st2, st3);
+  double* __restrict__ out = outbase + tk.out;
   if (!any) {
-    for (int k = tl; k < NAo * NBo * NCo * NDo; k += TE) {
+    for (int k = r0; k < NAo * NBo * NCo * NDo; k += rs) {
       const int ao = k % NAo, bo = (k / NAo) % NBo, co = (k / (NAo * NBo)) % NCo;
       const int dd = k / (NAo * NBo * NCo);
       out[ao * st0 + bo * st1 + co * st2 + dd * st3] = 0.0;
@@ -548,7 +556,7 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
     cdp[d][0] = 1.0;
     sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
   });
-  for (int row = tl; row < NAo * NBo; row += TE) {
+  for (int row = r0; row < NAo * NBo; row += rs) {
     const int ao = row / NBo, bo = row % NBo;
     const double* yr = Ys + row * NF;
     double Zc[NC * ND];
@@ -579,6 +587,37 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
       orow[co * st2 + dd * st3] = s;
     });
   }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD>;
+  using P = EpiCfg<LA, LB, LC, LD>;
+  constexpr int NEF = P::NEF, TE = P::TE;
+  __shared__ double sm[P::TPB * P::PER];
+  const int team = threadIdx.x / TE;
+  const int tl = threadIdx.x % TE;
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * P::TPB + team;
+  const bool valid = t < A.t1;
+  double* Ys = sm + team * P::PER + (P::XG ? 0 : NEF);
+  const double* X = sm + team * P::PER;
+  int any = 0;
+  Task tk;
+  if (valid) {
+    tk = A.tasks[t];
+    const int64_t slot = t - A.t0;
+    any = A.flags[slot];
+    const double* src = A.scratch + slot * NEF;
+    if constexpr (P::XG) {
+      X = src;
+    } else if (any) {
+      for (int k = tl; k < NEF; k += TE) sm[team * P::PER + k] = src[k];
+    }
+  }
+  __syncthreads();
+  if (valid && any) epi_bra<LA, LB, LC, LD>(Bv, tk, X, Ys, tl, TE);
+  __syncthreads();
+  if (valid) epi_ket_store<LA, LB, LC, LD>(Bv, tk, A.perm, Ys, A.out, any, tl, TE);
 }
 
 // ---------------------------------------------------------------------
@@ -791,6 +830,14 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
 
 template <int LA, int LB, int LC, int LD, int CH, int DER>
 hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t stream, dim3 grid) {
+  // the FP32 tier exists for the derivative chunks (gradient digest)
+  if constexpr (DER == 1) {
+    if (A.fp32) {
+      hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER, float>), grid, dim3(256), 0,
+                         stream, B, A);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER>), grid, dim3(256), 0, stream, B, A);
   return hipGetLastError();
 }
@@ -820,18 +867,19 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
 // Classes with a single-chunk fast kernel (canonical: LA>=LB, LC>=LD,
-// (LA,LB)>=(LC,LD), [e0|f0] block of at most 160 doubles), spherical output
-// of plain 4c ERIs.  The list is split in groups, one translation unit each
+// (LA,LB)>=(LC,LD), [e0|f0] block of at most 100 doubles, beyond which the
+// accumulators no longer fit the register file, and (fd|ps) at 138, measured
+// faster whole than chunked), spherical output of plain 4c ERIs.  The list is split in groups, one translation unit each
 // (fast_<g>.hip).
 #define MCINT_FAST_GROUP0(X) X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(2, 0, 0, 0) X(3, 0, 0, 0)
 #define MCINT_FAST_GROUP1(X) X(1, 1, 1, 0) X(2, 0, 1, 0) X(2, 1, 0, 0) X(2, 2, 0, 0) X(3, 1, 0, 0)
 #define MCINT_FAST_GROUP2(X) X(1, 1, 1, 1) X(3, 0, 1, 0) X(3, 2, 0, 0)
-#define MCINT_FAST_GROUP3(X) X(2, 0, 1, 1) X(2, 1, 1, 0) X(3, 3, 0, 0)
-#define MCINT_FAST_GROUP4(X) X(2, 1, 1, 1) X(2, 0, 2, 0)
+#define MCINT_FAST_GROUP3(X) X(2, 0, 1, 1) X(2, 1, 1, 0)
+#define MCINT_FAST_GROUP4(X) X(2, 0, 2, 0)
 #define MCINT_FAST_GROUP5(X) X(2, 1, 2, 0) X(3, 0, 1, 1)
 #define MCINT_FAST_GROUP6(X) X(2, 2, 1, 0) X(3, 0, 2, 0) X(3, 1, 1, 0)
-#define MCINT_FAST_GROUP7(X) X(3, 0, 2, 1) X(3, 0, 3, 0)
-#define MCINT_FAST_GROUP8(X) X(3, 1, 2, 0) X(3, 2, 1, 0)
+#define MCINT_FAST_GROUP7(X) X(3, 0, 3, 0)
+#define MCINT_FAST_GROUP8(X) X(3, 3, 0, 0) X(3, 2, 1, 0)
 #define MCINT_FAST_CLASSES(X)                                                         \
   MCINT_FAST_GROUP0(X) MCINT_FAST_GROUP1(X) MCINT_FAST_GROUP2(X) MCINT_FAST_GROUP3(X) \
   MCINT_FAST_GROUP4(X) MCINT_FAST_GROUP5(X) MCINT_FAST_GROUP6(X) MCINT_FAST_GROUP7(X) \
@@ -858,12 +906,17 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
 #define MCINT_CHUNK_GROUP14(X) X(3, 3, 3, 1)
 #define MCINT_CHUNK_GROUP15(X) X(3, 2, 2, 2) X(2, 2, 2, 2)
 #define MCINT_CHUNK_GROUP16(X) X(3, 3, 2, 1) X(3, 2, 3, 1)
+#define MCINT_CHUNK_GROUP17(X) X(3, 3, 3, 3)
+// accumulator blocks of 144..160 doubles: two chunks rather than a spilling
+// single kernel
+#define MCINT_CHUNK_GROUP18(X) X(2, 1, 1, 1) X(3, 1, 2, 0)
+#define MCINT_CHUNK_GROUP19(X) X(3, 0, 2, 1)
 #define MCINT_CHUNK_CLASSES(X)                                                              \
   MCINT_CHUNK_GROUP0(X) MCINT_CHUNK_GROUP1(X) MCINT_CHUNK_GROUP2(X) MCINT_CHUNK_GROUP3(X)   \
   MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X)   \
   MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X) MCINT_CHUNK_GROUP11(X) \
   MCINT_CHUNK_GROUP12(X) MCINT_CHUNK_GROUP13(X) MCINT_CHUNK_GROUP14(X) MCINT_CHUNK_GROUP15(X) \
-  MCINT_CHUNK_GROUP16(X)
+  MCINT_CHUNK_GROUP16(X) MCINT_CHUNK_GROUP17(X) MCINT_CHUNK_GROUP18(X) MCINT_CHUNK_GROUP19(X)
 
 // i-derivative (int2e_ip1) classes on the chunked path: shells up to d,
 // canonical LC >= LD (the derivative shell i stays first), one translation

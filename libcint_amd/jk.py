@@ -146,15 +146,29 @@ class GradientPlan:
     """Device plans of the ip1 list (i, j, k >= l) of a basis, reusable for
     any density: ``run(dm)`` returns dE/dR (natm x 3) of E = 1/2 D J - 1/4 D K."""
 
-    def __init__(self, basis: Basis, natm: int, chunk: int = 64_000_000, device="cuda"):
+    def __init__(self, basis: Basis, natm: int, chunk: int = 64_000_000, device="cuda",
+                 fp32_threshold: float = 0.0, dmax: float = 1.0):
         self.dev = torch.device(device)
         self.natm = natm
         self.nao = nao_sph(basis)
         self.db = DeviceBasis(basis)
         q_all = gradient_quartets(basis.nbas)
         self.nquartets = q_all.shape[0]
-        self.plans = [Plan(self.db, "int2e_ip1_sph", q_all[c0:c0 + chunk])
+        flags = None
+        if fp32_threshold > 0:
+            # the digest weighs each quartet by products of two density
+            # elements: FP32 where sqrt((ij|ij)) sqrt((kl|kl)) dmax^2 is small
+            flags = fp32_quartet_flags(self.db, q_all, np.array([dmax * dmax]), fp32_threshold)
+        self.plans = [Plan(self.db, "int2e_ip1_sph", q_all[c0:c0 + chunk],
+                           fp32_flags=None if flags is None else flags[c0:c0 + chunk])
                       for c0 in range(0, q_all.shape[0], chunk)]
+
+    def stats(self) -> dict:
+        out: dict = {}
+        for p in self.plans:
+            for k, v in p.stats().items():
+                out[k] = out.get(k, 0) + v
+        return out
 
     def run(self, dm, stream_ptr: int = 0) -> torch.Tensor:
         dm_t = torch.as_tensor(dm, dtype=torch.float64, device=self.dev).contiguous()
@@ -171,11 +185,16 @@ class GradientPlan:
         self.db.close()
 
 
-def nuclear_gradient(basis: Basis, dm, natm: int, device="cuda"):
-    """dE/dR (natm x 3, torch float64) of E = 1/2 D J - 1/4 D K."""
-    gp = GradientPlan(basis, natm, device=device)
+def nuclear_gradient(basis: Basis, dm, natm: int, device="cuda", fp32_threshold: float = 0.0):
+    """dE/dR (natm x 3, torch float64) of E = 1/2 D J - 1/4 D K.
+
+    fp32_threshold > 0: quartets with Schwarz bound x max|D|^2 below it run
+    their derivative recurrences in FP32 (the digest stays FP64)."""
+    dm = np.asarray(dm, dtype=np.float64)
+    gp = GradientPlan(basis, natm, device=device, fp32_threshold=fp32_threshold,
+                      dmax=float(np.abs(dm).max()))
     try:
-        grad = gp.run(np.asarray(dm, dtype=np.float64))
+        grad = gp.run(dm)
         torch.cuda.synchronize(gp.dev)
     finally:
         gp.close()

@@ -165,3 +165,21 @@ def test_mixed_precision_jk(c0_setup):
     _say(f"max|dJ| {dj:.2e} max|dK| {dk:.2e} dE {de:.2e}")
     assert dj < 1e-8 and dk < 1e-8
     assert abs(de) < 1e-9
+
+
+def test_mixed_precision_gradient():
+    """FP32 tier of the gradient digest (Schwarz x max|D|^2 < 1e-6): within
+    1e-8 Ha/Bohr of the FP64 gradient."""
+    import libcint_amd as mc
+    from libcint_amd import jk
+    m, dm = _small(6)
+    B = mc.Basis(m.atm, m.bas, m.env)
+    g64 = jk.nuclear_gradient(B, dm, m.natm).cpu().numpy()
+    gp = jk.GradientPlan(B, m.natm, fp32_threshold=1e-6, dmax=float(np.abs(dm).max()))
+    st = gp.stats()
+    g32 = gp.run(dm).cpu().numpy()
+    gp.close()
+    err = float(np.abs(g32 - g64).max())
+    _say("mixed gradient", st["fp32_tasks"], "of", st["tasks"], "FP32 tasks; max|dg|", err)
+    assert st["fp32_tasks"] > 0
+    assert err < 1e-8
