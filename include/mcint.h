@@ -145,6 +145,14 @@ int int2e_ip1_sph_batch(double *out, const size_t *out_offsets, const FINT *shls
 int int3c2e_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
                       const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
                       const CINTOpt *opt, FINT *not0, void *hip_stream);
+/* Cartesian twins of the two above (loops over int2e_ip1_cart,
+ * src/autocode/grad2.c:73-80, and int3c2e_cart, src/cint3c2e.c:709-716). */
+int int2e_ip1_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                         const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                         const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int3c2e_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                       const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                       const CINTOpt *opt, FINT *not0, void *hip_stream);
 
 /* ---- 3. device-resident plans ------------------------------------------- */
 typedef struct mcint_basis mcint_basis;

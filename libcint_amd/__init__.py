@@ -59,7 +59,8 @@ def lib():
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, vp, dp]
-    for name in ("int2e_sph_batch", "int2e_cart_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch"):
+    for name in ("int2e_sph_batch", "int2e_cart_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch",
+                 "int2e_ip1_cart_batch", "int3c2e_cart_batch"):
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [dp, ctypes.POINTER(ctypes.c_size_t), ip, ctypes.c_size_t, ip, ctypes.c_int,

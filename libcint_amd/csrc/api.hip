@@ -1027,6 +1027,22 @@ extern "C" int int3c2e_sph_batch(double* out, const size_t* out_offsets, const F
   return host_batch(MCINT_INT3C2E_SPH, out, out_offsets, shls, nq, atm, natm, bas, nbas, env, not0,
                     stream);
 }
+extern "C" int int2e_ip1_cart_batch(double* out, const size_t* out_offsets, const FINT* shls,
+                                    size_t nq, const FINT* atm, FINT natm, const FINT* bas,
+                                    FINT nbas, const double* env, const CINTOpt* opt, FINT* not0,
+                                    void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT2E_IP1_CART, out, out_offsets, shls, nq, atm, natm, bas, nbas, env,
+                    not0, stream);
+}
+extern "C" int int3c2e_cart_batch(double* out, const size_t* out_offsets, const FINT* shls,
+                                  size_t nq, const FINT* atm, FINT natm, const FINT* bas,
+                                  FINT nbas, const double* env, const CINTOpt* opt, FINT* not0,
+                                  void* stream) {
+  (void)opt;
+  return host_batch(MCINT_INT3C2E_CART, out, out_offsets, shls, nq, atm, natm, bas, nbas, env,
+                    not0, stream);
+}
 
 // ---------------------------------------------------------------------------
 // libcint-named per-call functions

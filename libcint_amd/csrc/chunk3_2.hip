@@ -1,0 +1,7 @@
+// Three-centre chunked-path instantiations, group 2 (see eri_fast.h).
+#include "device_common.h"
+#include "eri_fast.h"
+
+namespace mcint {
+MCINT_CHUNK3_GROUP2(MCINT_CHUNK3_DEFINE)
+}  // namespace mcint

@@ -83,22 +83,25 @@ __host__ __device__ constexpr double c2sc(int l, int m, int k) {
 // set (shells LA+1..E, each primitive weighted by 2 a_i) followed by the
 // lowered set (shells LA-1..E-2, absent for LA = 0), from which the
 // epilogue forms (d/dA_x a b| = a_x (a-1_x, b| - 2 a_i (a+1_x, b|.
+// DER = 2: three-centre (ij|k) = (ij|k s0) (src/g3c2e.c:21-132): the aux
+// shell k sits on C with LC = l_k, LD = 0 is an s function of exponent 0.
 template <int LA, int LB, int LC, int LD, int DER = 0>
 struct FastClass {
-  static constexpr int E = LA + LB + DER;
+  static constexpr int D1 = DER == 1 ? 1 : 0;
+  static constexpr int E = LA + LB + D1;
   static constexpr int F = LC + LD;
   static constexpr int NR = (E + F) / 2 + 1;
-  static constexpr int NEP = DER ? csum(LA + 1, E) : csum(LA, E);
-  static constexpr int NEM = (DER && LA > 0) ? csum(LA - 1, E - 2) : 0;
+  static constexpr int NEP = D1 ? csum(LA + 1, E) : csum(LA, E);
+  static constexpr int NEM = (D1 && LA > 0) ? csum(LA - 1, E - 2) : 0;
   static constexpr int NE = NEP + NEM;
   static constexpr int NF = csum(LC, F);
   static constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
   static constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
   static constexpr int NAo = 2 * LA + 1, NBo = 2 * LB + 1, NCo = 2 * LC + 1, NDo = 2 * LD + 1;
   // first shell of the list that row i belongs to, and its index there
-  static constexpr int row_lo(int i) { return DER ? (i < NEP ? LA + 1 : LA - 1) : LA; }
+  static constexpr int row_lo(int i) { return D1 ? (i < NEP ? LA + 1 : LA - 1) : LA; }
   static constexpr int row_k(int i) { return i < NEP ? i : i - NEP; }
-  static constexpr bool row_scaled(int i) { return DER && i < NEP; }
+  static constexpr bool row_scaled(int i) { return D1 && i < NEP; }
 };
 
 // Compile-time loop: f(std::integral_constant<int, 0..N-1>) expanded in place,
@@ -141,8 +144,7 @@ __device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, 
 // index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
 // is the team's pair-emptiness verdict, 'any' whether this lane saw a
 // primitive quartet that passed libcint's two screening tests.
-template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0,
-          int RL = 0>
+template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
                                            const double* __restrict__ rtab, R (&acc)[K1 - K0],
                                            int& any, int& live) {
@@ -154,27 +156,33 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   const VShell& sd = Bv.vsh[tk.sh[3]];
   const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
+  constexpr bool THREE = DER == 2;
   const int nij = sa.nprim * sb.nprim;
-  const int nkl = sc.nprim * sd.nprim;
+  const int nkl = THREE ? sc.nprim : sc.nprim * sd.nprim;
   const int64_t pab = (int64_t)tk.sh[0] * Bv.nv + tk.sh[1];
   const int64_t pcd = (int64_t)tk.sh[2] * Bv.nv + tk.sh[3];
   const PairPrim* __restrict__ pij = Bv.pp + Bv.pair_off[pab];
-  const PairPrim* __restrict__ pkl = Bv.pp + Bv.pair_off[pcd];
-  const double ld_ij = Bv.pair_ld[pab] * (LA + LB + DER);
-  const double ld_kl = Bv.pair_ld[pcd] * (LC + LD);
-  const double Ecut = Bv.expcutoff_4c;
-  // 2 pi^3 / sqrt(pi) times the s/p angular factors (src/g1e.c:565-572)
+  const PairPrim* __restrict__ pkl = THREE ? nullptr : Bv.pp + Bv.pair_off[pcd];
+  const double ld_ij = Bv.pair_ld[pab] * (LA + LB + C::D1);
+  const double ld_kl = THREE ? 0.0 : Bv.pair_ld[pcd] * (LC + LD);
+  const double Ecut = THREE ? Bv.expcutoff_3c : Bv.expcutoff_4c;
+  const double* __restrict__ ek = Bv.env + sc.pexp;   // aux exponents (DER = 2)
+  const double* __restrict__ ck = Bv.env + sc.pcoef;  // aux coefficients (DER = 2)
+  // 2 pi^3 / sqrt(pi) times the s/p angular factors (src/g1e.c:565-572); the
+  // exponent-0 s function of the three-centre case carries none
   constexpr double cfac = 34.986836655249725 *
                           (LA == 0 ? 0.282094791773878143 : LA == 1 ? 0.488602511902919921 : 1.0) *
                           (LB == 0 ? 0.282094791773878143 : LB == 1 ? 0.488602511902919921 : 1.0) *
                           (LC == 0 ? 0.282094791773878143 : LC == 1 ? 0.488602511902919921 : 1.0) *
-                          (LD == 0 ? 0.282094791773878143 : LD == 1 ? 0.488602511902919921 : 1.0);
+                          (THREE ? 1.0 : LD == 0 ? 0.282094791773878143
+                                         : LD == 1 ? 0.488602511902919921 : 1.0);
 
   // pair emptiness: libcint returns early when a pair has no primitive below
-  // the cutoff (src/cint2e.c:96-109)
-  int live_ij = 0, live_kl = 0;
+  // the cutoff (src/cint2e.c:96-109); the three-centre ket has no pair
+  int live_ij = 0, live_kl = THREE ? 1 : 0;
   for (int p = tl; p < nij; p += T) live_ij |= (pij[p].cce0 - ld_ij) < Ecut;
-  for (int p = tl; p < nkl; p += T) live_kl |= (pkl[p].cce0 - ld_kl) < Ecut;
+  if constexpr (!THREE)
+    for (int p = tl; p < nkl; p += T) live_kl |= (pkl[p].cce0 - ld_kl) < Ecut;
   for (int off = 1; off < T; off <<= 1) {
     live_ij |= __shfl_xor(live_ij, off);
     live_kl |= __shfl_xor(live_kl, off);
@@ -188,7 +196,18 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   int kl = tl / nij;
   int ij = tl - kl * nij;
   for (int p = tl; p < np; p += T) {
-    const PairPrim Q = pkl[kl];
+    PairPrim Q;
+    if constexpr (THREE) {
+      // ket "pair" of aux primitive kl and the exponent-0 s function
+      Q.px = Cx;
+      Q.py = Cy;
+      Q.pz = Cz;
+      Q.aij = ek[kl];
+      Q.kab = ck[kl];
+      Q.cce0 = 0.0;
+    } else {
+      Q = pkl[kl];
+    }
     const PairPrim P = pij[ij];
     // advance (ij fastest) before any 'continue'
     ij += T;
@@ -239,20 +258,9 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
           acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
       });
     };
-    const double x = a0 * rr;
-    if constexpr (RL) {
-      // rolled root loop: one root's state live at a time
-#pragma nounroll
-      for (int r = 0; r < NR; ++r) {
-        double ur, wr;
-        rys_root_sel<NR>(x, r, ur, wr, rtab);
-        root_body(ur, wr);
-      }
-    } else {
-      double u[NR], w[NR];
-      rys_roots<NR>(x, u, w, rtab);
-      sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
-    }
+    double u[NR], w[NR];
+    rys_roots<NR>(a0 * rr, u, w, rtab);
+    sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
   }
 }
 
@@ -277,9 +285,9 @@ __device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& t
   st3 = so[(perm >> 6) & 3];
 }
 
-template <int LA, int LB, int LC, int LD, typename R = double, int RL = 0>
+template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0>
 __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
-  using C = FastClass<LA, LB, LC, LD>;
+  using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int NE = C::NE, NF = C::NF;
   __shared__ double s_rys[RysLds<C::NR>::N];
   const double* rtab = rys_stage_table<C::NR>(s_rys);
@@ -297,7 +305,7 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
   R acc[NE * NF];
   int any, live;
-  accumulate<LA, LB, LC, LD, 0, NE * NF, R, 0, RL>(Bv, tk, T, tl, rtab, acc, any, live);
+  accumulate<LA, LB, LC, LD, 0, NE * NF, R, DER>(Bv, tk, T, tl, rtab, acc, any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -851,7 +859,7 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
   hipError_t e = hipSuccess;
   ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
-  if constexpr (DER == 0) {
+  if constexpr (DER != 1) {
     using P = EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
     hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
@@ -942,10 +950,35 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
   MCINT_IP1_GROUP8(X) MCINT_IP1_GROUP9(X) MCINT_IP1_GROUP10(X) MCINT_IP1_GROUP11(X) \
   MCINT_IP1_GROUP12(X) MCINT_IP1_GROUP13(X) MCINT_IP1_GROUP14(X)
 
+// Three-centre (ij|k) classes (LA >= LB, aux l_k = LC, LD = 0): single
+// kernel for blocks <= 100 doubles, chunked above; one translation unit per
+// group (fast3_<g>.hip, chunk3_<g>.hip).
+#define MCINT_FAST3_GROUP0(X) X(0, 0, 0, 0) X(0, 0, 1, 0) X(0, 0, 2, 0) X(0, 0, 3, 0) X(0, 0, 4, 0) \
+  X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 0, 2, 0)
+#define MCINT_FAST3_GROUP1(X) X(1, 0, 3, 0) X(1, 0, 4, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 2, 0) \
+  X(1, 1, 3, 0) X(2, 0, 0, 0) X(2, 0, 1, 0)
+#define MCINT_FAST3_GROUP2(X) X(2, 0, 2, 0) X(2, 0, 3, 0) X(2, 0, 4, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) \
+  X(2, 1, 2, 0) X(2, 2, 0, 0) X(2, 2, 1, 0)
+#define MCINT_FAST3_GROUP3(X) X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 2, 0) X(3, 0, 3, 0) X(3, 1, 0, 0) \
+  X(3, 1, 1, 0) X(3, 2, 0, 0) X(3, 3, 0, 0)
+#define MCINT_FAST3_CLASSES(X) \
+  MCINT_FAST3_GROUP0(X) MCINT_FAST3_GROUP1(X) MCINT_FAST3_GROUP2(X) MCINT_FAST3_GROUP3(X)
+#define MCINT_CHUNK3_GROUP0(X) X(3, 3, 4, 0) X(1, 1, 4, 0) X(2, 1, 3, 0) X(2, 1, 4, 0)
+#define MCINT_CHUNK3_GROUP1(X) X(3, 3, 3, 0) X(3, 2, 4, 0) X(3, 2, 1, 0)
+#define MCINT_CHUNK3_GROUP2(X) X(3, 3, 2, 0) X(3, 2, 3, 0) X(2, 2, 4, 0) X(2, 2, 2, 0) X(3, 0, 4, 0)
+#define MCINT_CHUNK3_GROUP3(X) X(3, 1, 4, 0) X(2, 2, 3, 0) X(3, 3, 1, 0) X(3, 1, 3, 0) \
+  X(3, 2, 2, 0) X(3, 1, 2, 0)
+#define MCINT_CHUNK3_CLASSES(X) \
+  MCINT_CHUNK3_GROUP0(X) MCINT_CHUNK3_GROUP1(X) MCINT_CHUNK3_GROUP2(X) MCINT_CHUNK3_GROUP3(X)
+
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_fast_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_fast3_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStream_t stream);
 
 // explicit instantiation of one class (used by fast_<g>.hip)
 #define MCINT_FAST_DEFINE(a, b, c, d)                                                      \
@@ -968,6 +1001,26 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
     return launch_chunks<a, b, c, d, 0>(                                                   \
         B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d>::NCH>{});       \
   }
+#define MCINT_FAST3_DEFINE(a, b, c, d)                                                     \
+  template <>                                                                              \
+  hipError_t launch_fast3_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,       \
+                                            hipStream_t stream) {                          \
+    const int64_t n = A.t1 - A.t0;                                                         \
+    const int64_t per = 256 / A.team;                                                      \
+    const dim3 grid((unsigned)((n + per - 1) / per));                                      \
+    if (A.fp32)                                                                            \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float, 2>), grid, dim3(256), 0, stream, B, A); \
+    else                                                                                   \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 2>), grid, dim3(256), 0, stream, B, A); \
+    return hipGetLastError();                                                              \
+  }
+#define MCINT_CHUNK3_DEFINE(a, b, c, d)                                                    \
+  template <>                                                                              \
+  hipError_t launch_chunk3_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,      \
+                                             hipStream_t stream) {                         \
+    return launch_chunks<a, b, c, d, 2>(                                                   \
+        B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d, 2>::NCH>{});    \
+  }
 #define MCINT_IP1_DEFINE(a, b, c, d)                                                       \
   template <>                                                                              \
   hipError_t launch_ip1_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,         \
@@ -988,6 +1041,14 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
   template <>                                                                              \
   hipError_t launch_ip1_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,         \
                                           hipStream_t stream);
+#define MCINT_FAST3_DECLARE(a, b, c, d)                                                    \
+  template <>                                                                              \
+  hipError_t launch_fast3_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,       \
+                                            hipStream_t stream);
+#define MCINT_CHUNK3_DECLARE(a, b, c, d)                                                   \
+  template <>                                                                              \
+  hipError_t launch_chunk3_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,      \
+                                             hipStream_t stream);
 
 // dispatch functions (expanded once, in eri.hip)
 #define MC_FAST_HAS(a, b, c, d) \
@@ -1004,14 +1065,32 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
   if (la == a && lb == b && lc == c && ld == d) return ChunkCfg<a, b, c, d, 1>::NEF;
 #define MC_IP1_LAUNCH(a, b, c, d) \
   if (A.la == a && A.lb == b && A.lc == c && A.ld == d) return launch_ip1_class<a, b, c, d>(B, A, stream);
+#define MC_FAST3_LAUNCH(a, b, c, d) \
+  if (A.la == a && A.lb == b && A.lc == c) return launch_fast3_class<a, b, c, d>(B, A, stream);
+#define MC_CHUNK3_LAUNCH(a, b, c, d) \
+  if (A.la == a && A.lb == b && A.lc == c) return launch_chunk3_class<a, b, c, d>(B, A, stream);
+#define MC_CHUNK3_NEF(a, b, c, d) \
+  if (la == a && lb == b && lc == c) return ChunkCfg<a, b, c, d, 2>::NEF;
+#define MC_FAST3_HAS(a, b, c, d) \
+  if (la == a && lb == b && lc == c) return 1;
+#define MC_CHUNK3_HAS(a, b, c, d) \
+  if (la == a && lb == b && lc == c) return 2;
 #define MCINT_FAST_DISPATCH                                                                \
   MCINT_FAST_CLASSES(MCINT_FAST_DECLARE)                                                   \
   MCINT_CHUNK_CLASSES(MCINT_CHUNK_DECLARE)                                                  \
   MCINT_IP1_CLASSES(MCINT_IP1_DECLARE)                                                     \
+  MCINT_FAST3_CLASSES(MCINT_FAST3_DECLARE)                                                 \
+  MCINT_CHUNK3_CLASSES(MCINT_CHUNK3_DECLARE)                                               \
   int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld) {                \
     if (!sph) return 0;                                                                    \
     if (kind == KIND_2E_IP1) {                                                             \
       MCINT_IP1_CLASSES(MC_CHUNK_HAS)                                                      \
+      return 0;                                                                            \
+    }                                                                                      \
+    if (kind == KIND_3C2E) {                                                               \
+      (void)ld;                                                                            \
+      MCINT_FAST3_CLASSES(MC_FAST3_HAS)                                                    \
+      MCINT_CHUNK3_CLASSES(MC_CHUNK3_HAS)                                                  \
       return 0;                                                                            \
     }                                                                                      \
     if (kind != KIND_2E) return 0;                                                         \
@@ -1024,6 +1103,10 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
       MCINT_IP1_CLASSES(MC_IP1_NEF)                                                        \
       return 0;                                                                            \
     }                                                                                      \
+    if (kind == KIND_3C2E) {                                                               \
+      MCINT_CHUNK3_CLASSES(MC_CHUNK3_NEF)                                                  \
+      return 0;                                                                            \
+    }                                                                                      \
     MCINT_CHUNK_CLASSES(MC_CHUNK_NEF)                                                      \
     return 0;                                                                              \
   }                                                                                        \
@@ -1031,6 +1114,10 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
   hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream) {    \
     if (A.t1 <= A.t0) return hipSuccess;                                                   \
     if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
+    if (A.kind == KIND_3C2E) {                                                             \
+      MCINT_FAST3_CLASSES(MC_FAST3_LAUNCH)                                                 \
+      return hipErrorNotSupported;                                                         \
+    }                                                                                      \
     MCINT_FAST_CLASSES(MC_FAST_LAUNCH)                                                     \
     return hipErrorNotSupported;                                                           \
   }                                                                                        \
@@ -1039,6 +1126,10 @@ hipError_t launch_ip1_class(const BasisView& B, const BucketArgs& A, hipStream_t
     if (A.team < 1 || A.team > 64 || (A.team & (A.team - 1))) return hipErrorInvalidValue; \
     if (A.kind == KIND_2E_IP1) {                                                           \
       MCINT_IP1_CLASSES(MC_IP1_LAUNCH)                                                     \
+      return hipErrorNotSupported;                                                         \
+    }                                                                                      \
+    if (A.kind == KIND_3C2E) {                                                             \
+      MCINT_CHUNK3_CLASSES(MC_CHUNK3_LAUNCH)                                               \
       return hipErrorNotSupported;                                                         \
     }                                                                                      \
     MCINT_CHUNK_CLASSES(MC_CHUNK_LAUNCH)                                                   \

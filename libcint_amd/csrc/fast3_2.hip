@@ -1,0 +1,7 @@
+// Three-centre single-kernel instantiations, group 2 (see eri_fast.h).
+#include "device_common.h"
+#include "eri_fast.h"
+
+namespace mcint {
+MCINT_FAST3_GROUP2(MCINT_FAST3_DEFINE)
+}  // namespace mcint

@@ -101,27 +101,4 @@ __device__ __forceinline__ void rys_roots(double x, double* u, double* w,
   }
 }
 
-// Root r of the NR-point rule only (runtime r).
-template <int NR>
-__device__ __forceinline__ void rys_root_sel(double x, int r, double& u, double& w,
-                                             const double* __restrict__ tab) {
-  if (x >= (double)RYS_XMAX(NR)) {
-    const double rr = rys_lag_r[rys_lag_offset[NR] + r];
-    u = rr * rcp_pos(x - rr);
-    w = rys_lag_w[rys_lag_offset[NR] + r] * rsqrt(x);
-  } else {
-    const int it = (int)x;
-    const double z = 2.0 * (x - (double)it) - 1.0;
-    const double* c = tab + it * (RYS_NCOEF * 2 * NR) + r;
-    double au = c[(RYS_NCOEF - 1) * 2 * NR], aw = c[(RYS_NCOEF - 1) * 2 * NR + NR];
-#pragma unroll
-    for (int j = RYS_NCOEF - 2; j >= 0; --j) {
-      au = fma(au, z, c[j * 2 * NR]);
-      aw = fma(aw, z, c[j * 2 * NR + NR]);
-    }
-    u = au;
-    w = aw;
-  }
-}
-
 }  // namespace mcint

@@ -114,7 +114,7 @@ class Basis:
 
 
 _KIND = {"int2e_sph": 0, "int2e_cart": 1, "int2e_ip1_sph": 2, "int2e_ip1_cart": 3,
-         "int3c2e_sph": 4}
+         "int3c2e_sph": 4, "int3c2e_cart": 5}
 
 
 def intor(name: str, shls, basis: Basis, dims=None):

@@ -1110,7 +1110,7 @@ int orc_int3c2e_cart(double *out, const int *dims, const int *shls, const int *a
 
 /* Batch driver shaped like examples/time_c2h6.c:719-745: OpenMP over a list
  * of shell quartets, dynamic schedule.  kind: 0 sph, 1 cart, 2 ip1_sph,
- * 3 ip1_cart, 4 3c2e_sph (shls stride 4, 4th ignored).  out[off[q]] receives
+ * 3 ip1_cart, 4 3c2e_sph, 5 3c2e_cart (shls stride 4, 4th ignored for 3c).  out[off[q]] receives
  * quartet q packed.  Returns the number of quartets whose status was < 0. */
 int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_t nq,
               const int *atm, int natm, const int *bas, int nbas, const double *env,
@@ -1129,7 +1129,8 @@ int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_
                 case 1: st = orc_int2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 2: st = orc_int2e_ip1_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 3: st = orc_int2e_ip1_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
-                default: st = orc_int3c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 4: st = orc_int3c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                default: st = orc_int3c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 }
                 if (not0) not0[q] = st;
                 if (st < 0) nbad++;

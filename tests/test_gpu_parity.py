@@ -55,7 +55,8 @@ def test_device_rys_roots(mc, orc):
 
 
 @pytest.mark.parametrize("name,key", [("int2e_sph", "cint2e_sph"), ("int2e_cart", None),
-                                      ("int2e_ip1_sph", "cint2e_ip1_sph")])
+                                      ("int2e_ip1_sph", "cint2e_ip1_sph"),
+                                      ("int2e_ip1_cart", None)])
 def test_libcint_fixture(mc, orc, golden, fixture_basis, name, key):
     """All (i<=j, k<=l) quartets of testsuite/test_cint.py's fixture: l <= 4,
     a 2-primitive 2-contraction d shell, up to 9 Rys roots."""
@@ -82,6 +83,52 @@ def test_libcint_fixture_3c2e(mc, orc, golden, fixture_basis):
     v = float(np.abs(got).sum())
     gref, place = golden["golden"]["cint3c2e_sph"]
     assert close_place(v, gref, got.size, place), (v, gref)
+
+
+def test_libcint_fixture_3c2e_cart(mc, orc, fixture_basis):
+    _say("fixture int3c2e_cart")
+    q = np.array([(i, j, k, 0) for k in range(8) for j in range(8) for i in range(8)], np.int32)
+    got, _, nz = mc.batch("int3c2e_cart", q, mc.Basis(*fixture_basis))
+    ref, _, nz_r = orc.batch("int3c2e_cart", q, orc.Basis(*fixture_basis))
+    _cmp(got, ref, "int3c2e_cart")
+    np.testing.assert_array_equal(nz, nz_r)
+
+
+def test_batch_with_caller_offsets(mc, fixture_basis):
+    """int2e_sph_batch with out_offsets (blocks placed with gaps, reverse
+    order) gives byte-identical blocks to the packed call; the gaps stay
+    untouched."""
+    import ctypes
+    _say("batch with caller offsets")
+    L = mc.lib()
+    q = libcint_quartets()[:200]
+    B = mc.Basis(*fixture_basis)
+    packed, off, _ = mc.batch("int2e_sph", q, B)
+    sizes = np.diff(off)
+    # reverse order with a 3-double gap before each block
+    pos = np.zeros(q.shape[0], dtype=np.uint64)
+    run = 0
+    for n in reversed(range(q.shape[0])):
+        run += 3
+        pos[n] = run
+        run += int(sizes[n])
+    out = np.full(run + 3, 7.25)
+    not0 = np.zeros(q.shape[0], dtype=np.int32)
+    shls = np.ascontiguousarray(q, dtype=np.int32)
+    rc = L.int2e_sph_batch(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           pos.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)),
+                           shls.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), q.shape[0],
+                           B.atm.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B.natm,
+                           B.bas.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B.nbas,
+                           B.env.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None,
+                           not0.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None)
+    assert rc == 0, mc.last_error()
+    touched = np.zeros(out.size, bool)
+    for n in range(q.shape[0]):
+        a = int(pos[n])
+        np.testing.assert_array_equal(out[a:a + sizes[n]], packed[off[n]:off[n + 1]])
+        touched[a:a + sizes[n]] = True
+    assert np.all(out[~touched] == 7.25)
 
 
 def _sample_quartets(nbas, n, seed):
@@ -273,3 +320,37 @@ def test_c4_3c2e_range_separated_sample(mc, orc, omega):
         orc.set_root_mode(orc.ROOTS_EXACT)
     _cmp(got, ref, f"c4 int3c2e_sph omega={omega}")
     np.testing.assert_array_equal(nz, nz_r)
+
+
+def test_c4_3c2e_fp32_tier(mc):
+    """FP32 tier of the three-centre path on a c4 sample: every quartet
+    flagged; the single-kernel classes run coefficients, recurrence and
+    contraction in FP32 (roots, weights and the result stay FP64).  Stated
+    error: max |FP32 - FP64| <= 1e-5 max|FP64| over the sample."""
+    import torch
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c4")
+    rng = np.random.default_rng(11)
+    n = 4000
+    q = np.zeros((n, 4), np.int32)
+    q[:, 0] = rng.integers(0, m.aux_start, n)
+    q[:, 1] = rng.integers(0, m.aux_start, n)
+    q[:, 2] = rng.integers(m.aux_start, m.nbas, n)
+    db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, m.env))
+    p64 = mc.Plan(db, "int3c2e_sph", q)
+    p32 = mc.Plan(db, "int3c2e_sph", q, fp32_flags=np.ones(n, np.uint8))
+    o64 = torch.zeros(p64.out_size, dtype=torch.float64, device="cuda")
+    o32 = torch.zeros(p32.out_size, dtype=torch.float64, device="cuda")
+    p64.execute(o64.data_ptr(), graph=False)
+    p32.execute(o32.data_ptr(), graph=False)
+    torch.cuda.synchronize()
+    st = p32.stats()
+    err = float((o32 - o64).abs().max())
+    scale = float(o64.abs().max())
+    _say("c4 3c2e FP32 tier:", st["fp32_tasks"], "of", st["tasks"], "tasks FP32; max|d|", err,
+         "max|ref|", scale)
+    assert st["fp32_tasks"] > 0
+    assert err <= 1e-5 * scale
+    p64.close()
+    p32.close()
+    db.close()
