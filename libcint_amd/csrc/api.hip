@@ -366,14 +366,17 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     const int la = l[L[0]], lb = l[L[1]], lc = l[L[2]], ld = kind == KIND_3C2E ? 0 : l[L[3]];
     const int64_t nprimq = (int64_t)np[0] * np[1] * np[2] * (naxes == 4 ? np[3] : 1);
     nprimq_total += nprimq;
-    // range-separated operators run on the generic kernel
-    const int mode = b->omega != 0 ? 0 : fast_kernel_mode(kind, sph, la, lb, lc, ld);
+    // range-separated operators run on the generic kernel, except the
+    // short-range three-centre integrals, which have register kernels
+    const bool reg_ok = b->omega == 0 || (b->omega < 0 && kind == KIND_3C2E);
+    const int mode = reg_ok ? fast_kernel_mode(kind, sph, la, lb, lc, ld) : 0;
     const int team = mode ? pick_team(nprimq) : 0;
     // FP32 recurrences where a single-chunk kernel or a derivative chunk
-    // kernel exists; other flagged quartets stay FP64
-    const uint32_t lowp =
-        (fp32_flags && fp32_flags[q] && (mode == 1 || (mode == 2 && kind == KIND_2E_IP1))) ? 1u
-                                                                                           : 0u;
+    // kernel exists (plain operator only); other flagged quartets stay FP64
+    const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 &&
+                           (mode == 1 || (mode == 2 && kind == KIND_2E_IP1)))
+                              ? 1u
+                              : 0u;
     int tl = 0;
     while ((1 << tl) < team) ++tl;
     const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |

@@ -85,6 +85,7 @@ __host__ __device__ constexpr double c2sc(int l, int m, int k) {
 // epilogue forms (d/dA_x a b| = a_x (a-1_x, b| - 2 a_i (a+1_x, b|.
 // DER = 2: three-centre (ij|k) = (ij|k s0) (src/g3c2e.c:21-132): the aux
 // shell k sits on C with LC = l_k, LD = 0 is an s function of exponent 0.
+// DER = 3: the same with the short-range operator erfc(|omega| r)/r.
 template <int LA, int LB, int LC, int LD, int DER = 0>
 struct FastClass {
   static constexpr int D1 = DER == 1 ? 1 : 0;
@@ -156,7 +157,8 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   const VShell& sd = Bv.vsh[tk.sh[3]];
   const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
-  constexpr bool THREE = DER == 2;
+  constexpr bool THREE = DER == 2 || DER == 3;
+  constexpr bool SR = DER == 3;  // three-centre erfc(omega r)/r, omega < 0
   const int nij = sa.nprim * sb.nprim;
   const int nkl = THREE ? sc.nprim : sc.nprim * sd.nprim;
   const int64_t pab = (int64_t)tk.sh[0] * Bv.nv + tk.sh[1];
@@ -177,10 +179,29 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
                           (THREE ? 1.0 : LD == 0 ? 0.282094791773878143
                                          : LD == 1 ? 0.488602511902919921 : 1.0);
 
+  // short range widens the distance term to log(d + theta r_guess + 1) and
+  // raises the loop cutoff (src/optimizer.c:301-315, src/cint3c2e.c:279-296;
+  // r_guess = 8); the pair-emptiness test keeps the plain cutoff
+  double ldij = ld_ij, Eloop = Ecut;
+  [[maybe_unused]] double w2 = 0.0;
+  if constexpr (SR) {
+    constexpr int lij = LA + LB, lk = LC;
+    w2 = Bv.omega * Bv.omega;
+    const double dab = sqrt((Ax - sb.r[0]) * (Ax - sb.r[0]) + (Ay - sb.r[1]) * (Ay - sb.r[1]) +
+                            (Az - sb.r[2]) * (Az - sb.r[2]));
+    const double th_ij =
+        w2 / (w2 + Bv.env[sa.pexp + sa.nprim - 1] + Bv.env[sb.pexp + sb.nprim - 1]);
+    ldij = lij > 0 ? lij * log(dab + th_ij * 8.0 + 1.0) : 0.0;
+    if (NR > 1) {
+      if (lij > 0) Eloop += lij * log((dab + th_ij * 8.0 + 1.0) / (dab + 1.0));
+      if (lk > 0) Eloop += lk * log(w2 / (w2 + ek[sc.nprim - 1]) * 8.0 + 1.0);
+    }
+  }
+
   // pair emptiness: libcint returns early when a pair has no primitive below
   // the cutoff (src/cint2e.c:96-109); the three-centre ket has no pair
   int live_ij = 0, live_kl = THREE ? 1 : 0;
-  for (int p = tl; p < nij; p += T) live_ij |= (pij[p].cce0 - ld_ij) < Ecut;
+  for (int p = tl; p < nij; p += T) live_ij |= (pij[p].cce0 - ldij) < Ecut;
   if constexpr (!THREE)
     for (int p = tl; p < nkl; p += T) live_kl |= (pkl[p].cce0 - ld_kl) < Ecut;
   for (int off = 1; off < T; off <<= 1) {
@@ -216,17 +237,28 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       ++kl;
     }
     const double cce_kl = Q.cce0 - ld_kl;
-    if (cce_kl > Ecut) continue;
-    const double cce_ij = P.cce0 - ld_ij;
-    if (cce_ij > Ecut - cce_kl) continue;
-    any = 1;
-    if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
+    if (cce_kl > Eloop) continue;
+    const double cce_ij = P.cce0 - ldij;
+    if (cce_ij > Eloop - cce_kl) continue;
 
     const double aij = P.aij, akl = Q.aij;
     const double a1 = aij * akl;
     const double a0 = a1 * rcp_pos(aij + akl);
     const double xpq = P.px - Q.px, ypq = P.py - Q.py, zpq = P.pz - Q.pz;
     const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
+    [[maybe_unused]] double theta = 0.0;
+    if constexpr (SR) {
+      // pairs beyond the plain cutoff carry no data (centre at 1e18), and
+      // negligible erfc tails are dropped (src/g2e.c:4449-4453, EXPCUTOFF_SR)
+      if (cce_ij >= Ecut) continue;
+      theta = w2 / (w2 + a0);
+      const double tx = theta * a0 * rr;
+      if (tx > Eloop - cce_kl - cce_ij || tx > 40.0) continue;
+      any = 1;
+    } else {
+      any = 1;
+      if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
+    }
     const double fac1 = sqrt(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
     const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
     const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
@@ -258,9 +290,25 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
           acc[k - K0] = fma(Vx[ex][fx] * Vy[ey][fy], Vz[ez][fz], acc[k - K0]);
       });
     };
-    double u[NR], w[NR];
-    rys_roots<NR>(a0 * rr, u, w, rtab);
-    sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+    if constexpr (SR) {
+      // erfc = 1 - erf: the full rule on NR roots and the theta-scaled rule
+      // on NR more, weights times -sqrt(theta) (src/g2e.c:4455-4476)
+      double u[2 * NR], w[2 * NR];
+      rys_roots<NR>(a0 * rr, u, w, rtab);
+      rys_roots<NR>(theta * a0 * rr, u + NR, w + NR, rtab);
+      const double mst = -sqrt(theta);
+      sfor<NR>([&](auto RI) {
+        constexpr int r = NR + decltype(RI)::value;
+        const double ut = u[r] * theta;
+        u[r] = ut * rcp_pos(u[r] + 1.0 - ut);
+        w[r] *= mst;
+      });
+      sfor<2 * NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+    } else {
+      double u[NR], w[NR];
+      rys_roots<NR>(a0 * rr, u, w, rtab);
+      sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+    }
   }
 }
 
@@ -1008,7 +1056,9 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
     const int64_t n = A.t1 - A.t0;                                                         \
     const int64_t per = 256 / A.team;                                                      \
     const dim3 grid((unsigned)((n + per - 1) / per));                                      \
-    if (A.fp32)                                                                            \
+    if (B.omega < 0)                                                                       \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 3>), grid, dim3(256), 0, stream, B, A); \
+    else if (A.fp32)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float, 2>), grid, dim3(256), 0, stream, B, A); \
     else                                                                                   \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 2>), grid, dim3(256), 0, stream, B, A); \
@@ -1018,6 +1068,9 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
   template <>                                                                              \
   hipError_t launch_chunk3_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,      \
                                              hipStream_t stream) {                         \
+    if (B.omega < 0)                                                                       \
+      return launch_chunks<a, b, c, d, 3>(                                                 \
+          B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d, 3>::NCH>{});  \
     return launch_chunks<a, b, c, d, 2>(                                                   \
         B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d, 2>::NCH>{});    \
   }

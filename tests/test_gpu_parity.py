@@ -301,11 +301,13 @@ def test_erf_plus_erfc_is_coulomb(mc, fixture_basis):
 def test_c4_3c2e_range_separated_sample(mc, orc, omega):
     """The 40-atom C/H cluster's (ij|P) with the erfc (omega = -0.3) and erf
     (omega = 0.3) operators against the oracle; aux shells up to g, so the
-    short-range rys_order reaches 6."""
+    short-range rys_order reaches 6.  The erfc case runs on the three-centre
+    register kernels (single-kernel and chunked classes), erf on the generic
+    kernel."""
     from libcint_amd.synthetic import make_molecule
     m = make_molecule("c4")
     rng = np.random.default_rng(7)
-    n = 400
+    n = 1500
     q = np.zeros((n, 4), np.int32)
     q[:, 0] = rng.integers(0, m.aux_start, n)
     q[:, 1] = rng.integers(0, m.aux_start, n)
