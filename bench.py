@@ -13,9 +13,11 @@ computed quartets / time), the definition of examples/time_c60.c:188,230.
 Also reported: the dominant bucket's algorithmic FLOP rate against MI355X's
 FP64 peak (HIP events on the plan's stream), the CPU oracle (this
 repository's C restatement of libcint, OpenMP over quartets) on a bounded
-random sample of the same quartets, and ("gradient") the J/K build and the
+random sample of the same quartets, ("gradient") the J/K build and the
 ip1 nuclear-gradient digest on the 20-atom C8H12 cluster timed against the
-plain FP64 int2e_sph pass over the same molecule.
+plain FP64 int2e_sph pass over the same molecule, and ("three_centre") the
+int3c2e batch of the 40-atom C16H24 cluster with its aux basis, FP64, FP32
+tier and short range.
 
     python bench.py [--steps K] [--warmup W] [--config c1]
 """
@@ -155,6 +157,66 @@ def gradient_section(reps: int = 3) -> dict:
     }
 
 
+def three_centre_section(reps: int = 3) -> dict:
+    """int3c2e_sph over every (i >= j | P) triple of the 40-atom C16H24
+    cluster (TZ-like orbital basis, aux shells up to g): FP64, the FP32 tier
+    for triples whose orbital pair has sqrt((ij|ij)) < 1e-4 (error against
+    FP64 stated), and the short-range erfc variant with omega = 0.3."""
+    import torch
+    import libcint_amd as mc
+    from libcint_amd.synthetic import make_molecule
+
+    m = make_molecule("c4")
+    ns = m.aux_start
+    ii, jj = np.tril_indices(ns)
+    aux = np.arange(ns, m.nbas, dtype=np.int32)
+    q = np.zeros((ii.size * aux.size, 4), np.int32)
+    q[:, 0] = np.repeat(ii, aux.size)
+    q[:, 1] = np.repeat(jj, aux.size)
+    q[:, 2] = np.tile(aux, ii.size)
+
+    def run(env, flags=None):
+        db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, env))
+        plan = mc.Plan(db, "int3c2e_sph", q, fp32_flags=flags)
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        plan.execute(out.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.execute(out.data_ptr())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        st = plan.stats()
+        plan.close()
+        db.close()
+        return dt, out, st
+
+    t64, o64, st = run(m.env)
+    db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, m.env))
+    qs = db.schwarz(ns)
+    db.close()
+    flags = (qs[q[:, 0], q[:, 1]] < 1e-4).astype(np.uint8)
+    t32, o32, st32 = run(m.env, flags)
+    err32 = float((o32 - o64).abs().max())
+    del o32
+    env_sr = m.env.copy()
+    env_sr[8] = -0.3
+    tsr, osr, _ = run(env_sr)
+    del osr
+    return {
+        "workload": f"c4: {m.name}, {ns} orbital + {m.nbas - ns} aux shells, every (i>=j|P)",
+        "triples": int(q.shape[0]),
+        "eris": int(st["out_size"]),
+        "fp64_ms": t64 * 1e3,
+        "fp64_eris_per_s": st["out_size"] / t64,
+        "fp32_tier_ms": t32 * 1e3,
+        "fp32_tier_triples": int(st32["fp32_tasks"]),
+        "fp32_tier_max_abs_error": err32,
+        "fp64_max_abs_value": float(o64.abs().max()),
+        "erfc_omega_0.3_ms": tsr * 1e3,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,7 +227,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-gradient", action="store_true",
-                    help="skip the J/K and nuclear-gradient timings on the C8H12 cluster")
+                    help="skip the J/K, nuclear-gradient and three-centre legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -314,6 +376,9 @@ def main():
         log("gradient / J/K legs ...")
         result["gradient"] = gradient_section()
         log(f"gradient: {result['gradient']}")
+        log("three-centre leg ...")
+        result["three_centre"] = three_centre_section()
+        log(f"three_centre: {result['three_centre']}")
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(B, quart, target_s=args.cpu_seconds)
