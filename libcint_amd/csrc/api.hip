@@ -17,7 +17,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mcint.h"
@@ -191,6 +194,105 @@ extern "C" void mcint_basis_destroy(mcint_basis* b) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Phase timing of plan construction, compiled in only with
+// -DMCINT_HOST_TIMING (stderr).
+#ifdef MCINT_HOST_TIMING
+struct PhaseTimer {
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char* what) {
+    const auto n = std::chrono::steady_clock::now();
+    fprintf(stderr, "[mcint plan] %-28s %8.3f s\n", what,
+            std::chrono::duration<double>(n - t).count());
+    t = n;
+  }
+};
+#else
+struct PhaseTimer {
+  void mark(const char*) {}
+};
+#endif
+
+// Host threads for plan construction: at most 16 (one GPU's share of a
+// node), one per 200k quartets.
+int host_threads(size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<size_t>(1, std::min<size_t>({(size_t)16, (size_t)hw, n / 200000 + 1}));
+}
+
+// f(thread, begin, end) over nthr contiguous ranges of [0, n)
+template <typename F>
+void parallel_for(int nthr, size_t n, F&& f) {
+  if (nthr <= 1) {
+    f(0, (size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nthr);
+  for (int t = 0; t < nthr; ++t) {
+    const size_t a = n * (size_t)t / (size_t)nthr, e = n * (size_t)(t + 1) / (size_t)nthr;
+    th.emplace_back([&f, t, a, e] { f(t, a, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Shell data of one quartet and its canonical axis order L[A] = original axis.
+struct QuartetShape {
+  int l[4], w[4], nc[4], np[4], L[4];
+  int64_t so[5];
+};
+
+// Set of bucket keys (open addressing; keys are < 2^27, ~0u marks empty).
+struct KeySet {
+  std::vector<uint32_t> slot = std::vector<uint32_t>(1024, ~0u);
+  size_t n = 0;
+  void insert(uint32_t k) {
+    if (2 * (n + 1) > slot.size()) grow();
+    size_t m = slot.size() - 1, h = (k * 2654435761u) & m;
+    while (slot[h] != ~0u && slot[h] != k) h = (h + 1) & m;
+    if (slot[h] == ~0u) {
+      slot[h] = k;
+      ++n;
+    }
+  }
+  void grow() {
+    std::vector<uint32_t> old;
+    old.swap(slot);
+    slot.assign(old.size() * 2, ~0u);
+    n = 0;
+    for (uint32_t k : old)
+      if (k != ~0u) insert(k);
+  }
+  std::vector<uint32_t> keys() const {
+    std::vector<uint32_t> v;
+    for (uint32_t k : slot)
+      if (k != ~0u) v.push_back(k);
+    return v;
+  }
+};
+
+// Key -> index in the sorted key list.
+struct KeyIndex {
+  std::vector<uint32_t> key;
+  std::vector<uint32_t> idx;
+  explicit KeyIndex(const std::vector<uint32_t>& sorted) {
+    size_t cap = 64;
+    while (cap < 2 * sorted.size() + 2) cap *= 2;
+    key.assign(cap, ~0u);
+    idx.assign(cap, 0);
+    for (size_t i = 0; i < sorted.size(); ++i) {
+      size_t h = (sorted[i] * 2654435761u) & (cap - 1);
+      while (key[h] != ~0u) h = (h + 1) & (cap - 1);
+      key[h] = sorted[i];
+      idx[h] = (uint32_t)i;
+    }
+  }
+  uint32_t find(uint32_t k) const {
+    size_t h = (k * 2654435761u) & (key.size() - 1);
+    while (key[h] != k) h = (h + 1) & (key.size() - 1);
+    return idx[h];
+  }
+};
+
 struct Bucket {
   int la, lb, lc, ld, perm, team;
   int mode;  // 0 generic, 1 fast (single chunk), 2 chunked
@@ -300,158 +402,235 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   p->ncomp = ncomp;
   p->nq = (int64_t)nq;
 
-  // ---- validate, offsets, canonical order, bucket keys -------------------
-  std::vector<int64_t> qoff(nq + 1, 0);
-  std::vector<uint32_t> key;  // per task
-  std::vector<Task> tasks;
-  std::vector<int> task_q;
-  tasks.reserve(nq);
-  key.reserve(nq);
-  task_q.reserve(nq);
-  int64_t running = 0;
-  int64_t nprimq_total = 0;
-  for (size_t q = 0; q < nq; ++q) {
+  // ---- per quartet: validation, block size, canonical order, bucket key --
+  // Threads take contiguous quartet ranges; tasks are then placed by a
+  // stable counting sort on the bucket, so the task order within a bucket
+  // is the list order.
+  PhaseTimer timer;
+  const int nthr = host_threads(nq);
+  // uninitialised: first touched by the worker threads
+  std::unique_ptr<uint32_t[]> qkey_buf(new uint32_t[nq + 1]);  // bucket key, then index
+  std::unique_ptr<uint32_t[]> qnt_buf(new uint32_t[nq + 1]);   // virtual tasks per quartet
+  std::unique_ptr<int64_t[]> qoff_buf(new int64_t[nq + 1]);
+  uint32_t* qkey = qkey_buf.get();
+  uint32_t* qnt = qnt_buf.get();
+  int64_t* qoff = qoff_buf.get();
+  qoff[0] = 0;
+  // path (0 generic, 1 single kernel, 2 chunked) of every class, once
+  std::vector<uint8_t> mode_tab((LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1), 0);
+  for (int la = 0; la <= LMAX; ++la)
+    for (int lb = 0; lb <= LMAX; ++lb)
+      for (int lc = 0; lc <= LMAX; ++lc)
+        for (int ld = 0; ld <= LMAX; ++ld)
+          mode_tab[((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld] =
+              (uint8_t)fast_kernel_mode(kind, sph, la, lb, lc, ld);
+  std::vector<int64_t> first_bad(nthr, INT64_MAX);
+  std::vector<int64_t> nprim_thr(nthr, 0), ntask_thr(nthr, 0);
+  const bool reg_ok = b->omega == 0 || (b->omega < 0 && kind == KIND_3C2E);
+  auto describe = [&](size_t q, QuartetShape& Q) -> int {
     const FINT* s = shls + 4 * q;
-    int l[4] = {0, 0, 0, 0}, w[4] = {1, 1, 1, 1}, nc[4] = {1, 1, 1, 1}, np[4] = {1, 1, 1, 1};
+    for (int a = 0; a < 4; ++a) {
+      Q.l[a] = 0;
+      Q.w[a] = Q.nc[a] = Q.np[a] = 1;
+    }
     for (int a = 0; a < naxes; ++a) {
-      if (s[a] < 0 || s[a] >= nbas) {
-        plan_free(p);
-        char msg[128];
-        snprintf(msg, sizeof msg, "mcint_plan_create: quartet %zu has shell index %d out of range",
-                 q, (int)s[a]);
-        set_err(msg);
-        return MCINT_ERR_ARG;
-      }
+      if (s[a] < 0 || s[a] >= nbas) return 1;
       const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
-      l[a] = bs[ANG_OF];
-      nc[a] = bs[NCTR_OF];
-      np[a] = bs[NPRIM_OF];
-      w[a] = sph ? nsph(l[a]) : ncart(l[a]);
+      Q.l[a] = bs[ANG_OF];
+      Q.nc[a] = bs[NCTR_OF];
+      Q.np[a] = bs[NPRIM_OF];
+      Q.w[a] = sph ? nsph(Q.l[a]) : ncart(Q.l[a]);
     }
-    const int lsum = l[0] + l[1] + l[2] + l[3] + (kind == KIND_2E_IP1 ? 1 : 0);
-    if (lsum / 2 + 1 > NRYS_MAX) {
-      plan_free(p);
-      char msg[160];
-      snprintf(msg, sizeof msg,
-               "mcint_plan_create: quartet %zu needs %d Rys roots; the device tables hold %d",
-               q, lsum / 2 + 1, NRYS_MAX);
-      set_err(msg);
-      return MCINT_ERR_UNSUPPORTED;
-    }
-    int64_t so[5];
-    so[0] = 1;
-    for (int o = 0; o < 4; ++o) so[o + 1] = so[o] * (int64_t)w[o] * nc[o];
-    const int64_t blk = so[4] * ncomp;
-    qoff[q] = out_offsets ? out_offsets[q] : running;
-    running += blk;
-
+    const int lsum = Q.l[0] + Q.l[1] + Q.l[2] + Q.l[3] + (kind == KIND_2E_IP1 ? 1 : 0);
+    if (lsum / 2 + 1 > NRYS_MAX) return 2;
+    Q.so[0] = 1;
+    for (int o = 0; o < 4; ++o) Q.so[o + 1] = Q.so[o] * (int64_t)Q.w[o] * Q.nc[o];
     // canonical axis order L[A] = original axis
-    int L[4] = {0, 1, 2, 3};
+    for (int a = 0; a < 4; ++a) Q.L[a] = a;
+    int* L = Q.L;
+    const int* l = Q.l;
     if (kind == KIND_2E) {
       if (l[L[0]] < l[L[1]]) std::swap(L[0], L[1]);
       if (l[L[2]] < l[L[3]]) std::swap(L[2], L[3]);
       if (l[L[2]] > l[L[0]] || (l[L[2]] == l[L[0]] && l[L[3]] > l[L[1]])) {
-        int t0 = L[0], t1 = L[1];
-        L[0] = L[2];
-        L[1] = L[3];
-        L[2] = t0;
-        L[3] = t1;
+        std::swap(L[0], L[2]);
+        std::swap(L[1], L[3]);
       }
     } else if (kind == KIND_2E_IP1) {
       if (l[2] < l[3]) std::swap(L[2], L[3]);
     } else {
       if (l[0] < l[1]) std::swap(L[0], L[1]);
     }
-    const int perm = L[0] | (L[1] << 2) | (L[2] << 4) | (L[3] << 6);
-    const int la = l[L[0]], lb = l[L[1]], lc = l[L[2]], ld = kind == KIND_3C2E ? 0 : l[L[3]];
-    const int64_t nprimq = (int64_t)np[0] * np[1] * np[2] * (naxes == 4 ? np[3] : 1);
-    nprimq_total += nprimq;
-    // range-separated operators run on the generic kernel, except the
-    // short-range three-centre integrals, which have register kernels
-    const bool reg_ok = b->omega == 0 || (b->omega < 0 && kind == KIND_3C2E);
-    const int mode = reg_ok ? fast_kernel_mode(kind, sph, la, lb, lc, ld) : 0;
-    const int team = mode ? pick_team(nprimq) : 0;
-    // FP32 recurrences where a single-chunk kernel or a derivative chunk
-    // kernel exists (plain operator only); other flagged quartets stay FP64
-    const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 &&
-                           (mode == 1 || (mode == 2 && kind == KIND_2E_IP1)))
-                              ? 1u
-                              : 0u;
-    int tl = 0;
-    while ((1 << tl) < team) ++tl;
-    const uint32_t k = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) |
-                       ((uint32_t)ld << 9) | ((uint32_t)perm << 12) | ((uint32_t)tl << 20) |
-                       ((uint32_t)mode << 24) | (lowp << 26);
-    // virtual expansion over contraction columns of the original axes
-    int cidx[4] = {0, 0, 0, 0};
-    for (;;) {
-      Task t;
-      int64_t o = qoff[q];
-      for (int a = 0; a < 4; ++a) {
-        if (a < naxes) o += (int64_t)cidx[a] * w[a] * so[a];
+    return 0;
+  };
+  parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
+    // per-thread totals kept in registers (the shared arrays are written
+    // once: adjacent slots would share a cache line)
+    int64_t bad = INT64_MAX, nprim_sum = 0, ntask_sum = 0;
+    QuartetShape Q;
+    for (size_t q = q0; q < q1; ++q) {
+      if (describe(q, Q)) {
+        bad = std::min<int64_t>(bad, (int64_t)q);
+        qkey[q] = 0;
+        qnt[q] = 0;
+        continue;
       }
-      for (int A = 0; A < 4; ++A) {
-        if (A < naxes) {
-          const int orig = L[A];
-          t.sh[A] = (int16_t)(b->vfirst[s[orig]] + cidx[orig]);
-        } else {
-          t.sh[A] = t.sh[naxes - 1];
-        }
-      }
-      t.out = o;
-      tasks.push_back(t);
-      key.push_back(k);
-      task_q.push_back((int)q);
-      int a = 0;
-      for (; a < naxes; ++a) {
-        if (++cidx[a] < nc[a]) break;
-        cidx[a] = 0;
-      }
-      if (a == naxes) break;
+      const int* L = Q.L;
+      const int perm = L[0] | (L[1] << 2) | (L[2] << 4) | (L[3] << 6);
+      const int la = Q.l[L[0]], lb = Q.l[L[1]], lc = Q.l[L[2]];
+      const int ld = kind == KIND_3C2E ? 0 : Q.l[L[3]];
+      const int64_t nprimq = (int64_t)Q.np[0] * Q.np[1] * Q.np[2] * (naxes == 4 ? Q.np[3] : 1);
+      nprim_sum += nprimq;
+      // range-separated operators run on the generic kernel, except the
+      // short-range three-centre integrals, which have register kernels
+      const int mode =
+          reg_ok ? mode_tab[((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld] : 0;
+      const int team = mode ? pick_team(nprimq) : 0;
+      // FP32 recurrences where a single-chunk kernel or a derivative chunk
+      // kernel exists (plain operator only); other flagged quartets stay FP64
+      const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 &&
+                             (mode == 1 || (mode == 2 && kind == KIND_2E_IP1)))
+                                ? 1u
+                                : 0u;
+      int tl = 0;
+      while ((1 << tl) < team) ++tl;
+      qkey[q] = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) | ((uint32_t)ld << 9) |
+                ((uint32_t)perm << 12) | ((uint32_t)tl << 20) | ((uint32_t)mode << 24) |
+                (lowp << 26);
+      uint32_t nt = 1;
+      for (int a = 0; a < naxes; ++a) nt *= (uint32_t)Q.nc[a];
+      qnt[q] = nt;
+      ntask_sum += nt;
+      qoff[q + 1] = Q.so[4] * ncomp;  // block size for now
     }
+    first_bad[th] = bad;
+    nprim_thr[th] = nprim_sum;
+    ntask_thr[th] = ntask_sum;
+  });
+  timer.mark(nthr > 1 ? "describe quartets (threads)" : "describe quartets (1 thread)");
+  int64_t bad_q = INT64_MAX;
+  for (int th = 0; th < nthr; ++th) bad_q = std::min(bad_q, first_bad[th]);
+  if (bad_q != INT64_MAX) {
+    QuartetShape Q;
+    const int why = describe((size_t)bad_q, Q);
+    char msg[192];
+    if (why == 1) {
+      snprintf(msg, sizeof msg, "mcint_plan_create: quartet %lld has a shell index out of range",
+               (long long)bad_q);
+      set_err(msg);
+      plan_free(p);
+      return MCINT_ERR_ARG;
+    }
+    snprintf(msg, sizeof msg,
+             "mcint_plan_create: quartet %lld needs more Rys roots than the device tables hold (%d)",
+             (long long)bad_q, NRYS_MAX);
+    set_err(msg);
+    plan_free(p);
+    return MCINT_ERR_UNSUPPORTED;
   }
-  p->out_size = out_offsets ? 0 : running;
+  int64_t nprimq_total = 0, ntasks = 0;
+  for (int th = 0; th < nthr; ++th) {
+    nprimq_total += nprim_thr[th];
+    ntasks += ntask_thr[th];
+  }
   if (out_offsets) {
     int64_t mx = 0;
-    for (size_t q = 0; q < nq; ++q) {
-      const FINT* s = shls + 4 * q;
-      int64_t blk = ncomp;
-      for (int a = 0; a < naxes; ++a) {
-        const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
-        blk *= (int64_t)(sph ? nsph(bs[ANG_OF]) : ncart(bs[ANG_OF])) * bs[NCTR_OF];
-      }
-      mx = std::max(mx, out_offsets[q] + blk);
-    }
+    for (size_t q = 0; q < nq; ++q) mx = std::max(mx, out_offsets[q] + qoff[q + 1]);
+    for (size_t q = 0; q < nq; ++q) qoff[q] = out_offsets[q];
     p->out_size = mx;
+  } else {
+    for (size_t q = 0; q < nq; ++q) qoff[q + 1] += qoff[q];
+    p->out_size = qoff[nq];
   }
   if (out_size) *out_size = p->out_size;
-  p->ntasks = (int64_t)tasks.size();
+  p->ntasks = ntasks;
   p->stats[4] = nprimq_total;
   p->stats[5] = p->out_size;
 
-  // ---- counting sort by bucket key ---------------------------------------
-  std::vector<uint32_t> ukeys(key);
-  std::sort(ukeys.begin(), ukeys.end());
-  ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
-  std::vector<int64_t> cnt(ukeys.size() + 1, 0);
-  std::vector<int> kidx(tasks.size());
-  for (size_t t = 0; t < tasks.size(); ++t) {
-    kidx[t] = (int)(std::lower_bound(ukeys.begin(), ukeys.end(), key[t]) - ukeys.begin());
-    cnt[kidx[t] + 1]++;
-  }
-  for (size_t i = 0; i < ukeys.size(); ++i) cnt[i + 1] += cnt[i];
-  std::vector<Task> sorted(tasks.size());
-  std::vector<int> sorted_q(tasks.size());
+  // ---- bucket index of every quartet: the distinct keys (a few hundred),
+  // sorted, behind a small open-addressing table
+  std::vector<uint32_t> ukeys;
   {
-    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-    for (size_t t = 0; t < tasks.size(); ++t) {
-      const int64_t d = pos[kidx[t]]++;
-      sorted[d] = tasks[t];
-      sorted_q[d] = task_q[t];
-    }
+    std::vector<std::vector<uint32_t>> tk(nthr);
+    parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
+      KeySet ks;
+      for (size_t q = q0; q < q1; ++q)
+        if (qnt[q]) ks.insert(qkey[q]);
+      tk[th] = ks.keys();
+    });
+    for (auto& v : tk) ukeys.insert(ukeys.end(), v.begin(), v.end());
+    std::sort(ukeys.begin(), ukeys.end());
+    ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
   }
+  timer.mark("offsets, distinct keys");
+  const KeyIndex kindex(ukeys);
+  const size_t nkeys = ukeys.size();
+  std::vector<int64_t> tcount((size_t)nthr * nkeys, 0);
+  parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
+    int64_t* c = tcount.data() + (size_t)th * nkeys;
+    for (size_t q = q0; q < q1; ++q) {
+      if (!qnt[q]) continue;
+      const uint32_t id = kindex.find(qkey[q]);
+      qkey[q] = id;
+      c[id] += qnt[q];
+    }
+  });
+  // bucket starts, and each thread's cursor inside every bucket
+  std::vector<int64_t> cnt(nkeys + 1, 0);
+  std::vector<int64_t> cursor((size_t)nthr * nkeys, 0);
+  {
+    int64_t run = 0;
+    for (size_t i = 0; i < nkeys; ++i) {
+      cnt[i] = run;
+      for (int th = 0; th < nthr; ++th) {
+        cursor[(size_t)th * nkeys + i] = run;
+        run += tcount[(size_t)th * nkeys + i];
+      }
+    }
+    cnt[nkeys] = run;
+  }
+  timer.mark("bucket counts");
+  // uninitialised: every slot is written by the placement pass
+  std::unique_ptr<Task[]> sorted(new Task[(size_t)std::max<int64_t>(ntasks, 1)]);
+  std::unique_ptr<int[]> sorted_q(new int[(size_t)std::max<int64_t>(ntasks, 1)]);
+  parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
+    int64_t* cur = cursor.data() + (size_t)th * nkeys;
+    QuartetShape Q;
+    for (size_t q = q0; q < q1; ++q) {
+      if (!qnt[q]) continue;
+      describe(q, Q);
+      const FINT* s = shls + 4 * q;
+      int64_t& d = cur[qkey[q]];
+      // virtual expansion over contraction columns of the original axes
+      int cidx[4] = {0, 0, 0, 0};
+      for (;;) {
+        Task t;
+        int64_t o = qoff[q];
+        for (int a = 0; a < naxes; ++a) o += (int64_t)cidx[a] * Q.w[a] * Q.so[a];
+        for (int A = 0; A < 4; ++A) {
+          if (A < naxes) {
+            const int orig = Q.L[A];
+            t.sh[A] = (int16_t)(b->vfirst[s[orig]] + cidx[orig]);
+          } else {
+            t.sh[A] = t.sh[naxes - 1];
+          }
+        }
+        t.out = o;
+        sorted[d] = t;
+        sorted_q[d] = (int)q;
+        ++d;
+        int a = 0;
+        for (; a < naxes; ++a) {
+          if (++cidx[a] < Q.nc[a]) break;
+          cidx[a] = 0;
+        }
+        if (a == naxes) break;
+      }
+    }
+  });
+  timer.mark("task placement");
   int64_t max_gen = 0, max_flags = 0;
-  for (size_t i = 0; i < ukeys.size(); ++i) {
+  for (size_t i = 0; i < nkeys; ++i) {
     const uint32_t k = ukeys[i];
     Bucket bk;
     bk.la = k & 7;
@@ -498,26 +677,34 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
 
   // ---- primitive-pair data of every pair the tasks touch ----------------
   const int nv = b->nv;
-  std::vector<int64_t> pair_off((size_t)nv * nv, -1);
+  std::vector<uint8_t> used((size_t)nv * nv, 0);
+  // read before writing: stores to lines other threads read would bounce
+  // the lines between cores
+  auto mark = [&](size_t i) {
+    if (!__atomic_load_n(&used[i], __ATOMIC_RELAXED)) __atomic_store_n(&used[i], (uint8_t)1, __ATOMIC_RELAXED);
+  };
+  parallel_for(nthr, (size_t)ntasks, [&](int, size_t t0, size_t t1) {
+    for (size_t t = t0; t < t1; ++t) {
+      const Task& tk = sorted[t];
+      mark((size_t)tk.sh[0] * nv + tk.sh[1]);
+      if (kind != KIND_3C2E) mark((size_t)tk.sh[2] * nv + tk.sh[3]);
+    }
+  });
+  timer.mark("buckets, pair marks");
+  std::vector<int64_t> pair_off((size_t)nv * nv, 0);
   std::vector<int> pairs;
   int64_t npp = 0;
-  auto touch = [&](int a, int c) {
-    int64_t& o = pair_off[(size_t)a * nv + c];
-    if (o >= 0) return;
-    o = npp;
-    npp += (int64_t)b->vsh[a].nprim * b->vsh[c].nprim;
-    pairs.push_back(a);
-    pairs.push_back(c);
-  };
-  for (const Task& t : sorted) {
-    touch(t.sh[0], t.sh[1]);
-    if (kind != KIND_3C2E) touch(t.sh[2], t.sh[3]);
-  }
-  for (auto& o : pair_off)
-    if (o < 0) o = 0;
+  for (int a = 0; a < nv; ++a)
+    for (int c = 0; c < nv; ++c) {
+      if (!used[(size_t)a * nv + c]) continue;
+      pair_off[(size_t)a * nv + c] = npp;
+      npp += (int64_t)b->vsh[a].nprim * b->vsh[c].nprim;
+      pairs.push_back(a);
+      pairs.push_back(c);
+    }
   int* pairs_d = nullptr;
-  hipError_t e = dalloc(&p->tasks_d, sorted.size());
-  if (e == hipSuccess) e = dalloc(&p->task_id_d, sorted_q.size());
+  hipError_t e = dalloc(&p->tasks_d, (size_t)std::max<int64_t>(ntasks, 1));
+  if (e == hipSuccess) e = dalloc(&p->task_id_d, (size_t)std::max<int64_t>(ntasks, 1));
   if (e == hipSuccess) e = dalloc(&p->pair_off_d, pair_off.size());
   if (e == hipSuccess) e = dalloc(&p->pair_ld_d, pair_off.size());
   if (e == hipSuccess) e = dalloc(&p->pp_d, (size_t)std::max<int64_t>(npp, 1));
@@ -526,10 +713,10 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   if (e == hipSuccess && max_flags > 0) e = dalloc(&p->flags_d, (size_t)max_flags);
   p->scratch_size = max_gen;
   p->flags_size = max_flags;
-  if (e == hipSuccess && !sorted.empty())
-    e = hipMemcpy(p->tasks_d, sorted.data(), sorted.size() * sizeof(Task), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !sorted_q.empty())
-    e = hipMemcpy(p->task_id_d, sorted_q.data(), sorted_q.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ntasks > 0)
+    e = hipMemcpy(p->tasks_d, sorted.get(), (size_t)ntasks * sizeof(Task), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ntasks > 0)
+    e = hipMemcpy(p->task_id_d, sorted_q.get(), (size_t)ntasks * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->pair_off_d, pair_off.data(), pair_off.size() * sizeof(int64_t),
                   hipMemcpyHostToDevice);
@@ -539,6 +726,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     e = launch_pairdata(b->vsh_d, b->env_d, nv, pairs_d, (int)(pairs.size() / 2), p->pair_off_d,
                         p->pair_ld_d, p->pp_d, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
+  timer.mark("device upload, pair data");
   dfree(pairs_d);
   if (e != hipSuccess) {
     set_err(std::string("mcint_plan_create: ") + hipGetErrorString(e));
