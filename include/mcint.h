@@ -251,6 +251,12 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0
 int mcint_tune(int key, int value);
+/* Per-class override of MCINT_TUNE_PRIMS_PER_LANE for one integral kind
+ * (intor code) and canonical class (la >= lb, lc >= ld, (la,lb) >= (lc,ld)
+ * for int2e; derivative shell first for ip1; ld = 0 for int3c2e); 0
+ * restores the global value.  Written by the autotuner
+ * (libcint_amd/autotune.py).  Returns the previous override. */
+int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int prims_per_lane);
 
 const char *mcint_last_error(void);
 

@@ -106,6 +106,8 @@ def lib():
     L.mcint_schwarz.argtypes = [vp, ctypes.c_int, dp]
     L.mcint_tune.restype = ctypes.c_int
     L.mcint_tune.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.mcint_tune_class.restype = ctypes.c_int
+    L.mcint_tune_class.argtypes = [ctypes.c_int] * 6
     L.mcint_last_error.restype = ctypes.c_char_p
     L.CINTrys_roots.restype = ctypes.c_int
     L.CINTrys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
@@ -114,6 +116,10 @@ def lib():
     L.CINTgto_norm.restype = ctypes.c_double
     L.CINTgto_norm.argtypes = [ctypes.c_int, ctypes.c_double]
     _lib = L
+    # per-class team sizes measured by the autotuner on MI355X
+    from . import autotune
+    if os.path.exists(autotune.DEFAULT_TABLE):
+        autotune.apply(autotune.load())
     return L
 
 
@@ -125,6 +131,14 @@ def tune(key: int, value: int) -> int:
     rc = lib().mcint_tune(key, value)
     if rc < 0:
         raise McintError(f"mcint_tune failed ({rc}): {last_error()}")
+    return rc
+
+
+def tune_class(name: str, cls, prims_per_lane: int) -> int:
+    """Per-class primitive quartets per lane (mcint_tune_class); 0 resets."""
+    rc = lib().mcint_tune_class(INTOR_CODES[name], *[int(c) for c in cls], int(prims_per_lane))
+    if rc < 0:
+        raise McintError(f"mcint_tune_class failed ({rc}): {last_error()}")
     return rc
 
 

@@ -310,10 +310,18 @@ constexpr int GRAD_SLOTS = 64;
 // Tuning state (mcint_tune): primitive quartets per lane that the team
 // size aims at.
 int g_prims_per_lane = 32;
+// per-class overrides (mcint_tune_class), 0 = the global value; indexed by
+// integral kind and canonical class
+constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
+int16_t g_class_ppl[3][NCLS] = {};
 
-int pick_team(int64_t nprim_quartets) {
+inline int class_index(int la, int lb, int lc, int ld) {
+  return ((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld;
+}
+
+int pick_team(int64_t nprim_quartets, int ppl) {
   int t = 1;
-  while (t < fast_team_lanes_max() && (int64_t)t * g_prims_per_lane < nprim_quartets) t *= 2;
+  while (t < fast_team_lanes_max() && (int64_t)t * ppl < nprim_quartets) t *= 2;
   return t;
 }
 
@@ -360,6 +368,19 @@ static void plan_free(mcint_plan* p) {
 }
 
 extern "C" void mcint_plan_destroy(mcint_plan* p) { plan_free(p); }
+
+extern "C" int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int prims_per_lane) {
+  const int kind = intor_kind(intor);
+  if (kind < 0 || la < 0 || lb < 0 || lc < 0 || ld < 0 || la > LMAX || lb > LMAX || lc > LMAX ||
+      ld > LMAX || prims_per_lane < 0 || prims_per_lane > 4096) {
+    set_err("mcint_tune_class: bad argument");
+    return MCINT_ERR_ARG;
+  }
+  int16_t& v = g_class_ppl[kind][class_index(la, lb, lc, ld)];
+  const int old = v;
+  v = (int16_t)prims_per_lane;
+  return old;
+}
 
 extern "C" int mcint_tune(int key, int value) {
   if (key == MCINT_TUNE_PRIMS_PER_LANE) {
@@ -485,7 +506,8 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       // short-range three-centre integrals, which have register kernels
       const int mode =
           reg_ok ? mode_tab[((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld] : 0;
-      const int team = mode ? pick_team(nprimq) : 0;
+      const int ppl_c = g_class_ppl[kind][class_index(la, lb, lc, ld)];
+      const int team = mode ? pick_team(nprimq, ppl_c ? ppl_c : g_prims_per_lane) : 0;
       // FP32 recurrences where a single-chunk kernel or a derivative chunk
       // kernel exists (plain operator only); other flagged quartets stay FP64
       const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 &&
