@@ -311,15 +311,16 @@ def main():
     dom_flops = bflops[dom]
     achieved = dom_flops / dom_t / 1e12
     total_flops = float(sum(bflops))
-    # HBM traffic of the dominant launch, when a committed counter profile
-    # (rocprofv3 FETCH_SIZE/WRITE_SIZE) matches this launch's kernel and grid
+    # HBM traffic of the dominant launch (all kernels of the bucket), when a
+    # committed counter profile (rocprofv3 FETCH_SIZE/WRITE_SIZE) matches this
+    # bucket's class, path, team size and task count
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "dominant_traffic.json")) as fh:
             tr = json.load(fh)
         b = buckets[dom]
-        grid = -(-b["tasks"] // (256 // b["team"])) * 256 if b["path"] == "fast" else None
-        if tuple(tr["cls"]) == tuple(b["cls"]) and tr["path"] == b["path"] and tr["grid_x"] == grid:
+        if (tuple(tr["cls"]) == tuple(b["cls"]) and tr["path"] == b["path"]
+                and tr["team"] == b["team"] and tr["tasks"] == b["tasks"]):
             traffic = tr["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         traffic = None
@@ -364,7 +365,8 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "kernel": f"class {buckets[dom]['cls']} team {buckets[dom]['team']} "
-                      f"({buckets[dom]['path']}), "
+                      f"({buckets[dom]['path']}"
+                      f"{': component-chunk kernels + LDS epilogue, timed together' if buckets[dom]['path'] == 'chunked' else ''}), "
                       f"{buckets[dom]['tasks']} quartets, {int(nprim[dom])} primitive quartets",
             "launch_ms": dom_t * 1e3,
             "flops_per_launch": dom_flops,

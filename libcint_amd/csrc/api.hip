@@ -337,6 +337,7 @@ struct mcint_plan {
   int64_t* pair_off_d = nullptr;
   double* pair_ld_d = nullptr;
   PairPrim* pp_d = nullptr;
+  int* pp_pos_d = nullptr;
   double* scratch_d = nullptr;
   int64_t scratch_size = 0;
   int* flags_d = nullptr;
@@ -361,6 +362,7 @@ static void plan_free(mcint_plan* p) {
   dfree(p->pair_off_d);
   dfree(p->pair_ld_d);
   dfree(p->pp_d);
+  dfree(p->pp_pos_d);
   dfree(p->scratch_d);
   dfree(p->flags_d);
   dfree(p->gslots_d);
@@ -725,11 +727,14 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       pairs.push_back(c);
     }
   int* pairs_d = nullptr;
+  PairPrim* pp_tmp = nullptr;
   hipError_t e = dalloc(&p->tasks_d, (size_t)std::max<int64_t>(ntasks, 1));
   if (e == hipSuccess) e = dalloc(&p->task_id_d, (size_t)std::max<int64_t>(ntasks, 1));
   if (e == hipSuccess) e = dalloc(&p->pair_off_d, pair_off.size());
   if (e == hipSuccess) e = dalloc(&p->pair_ld_d, pair_off.size());
   if (e == hipSuccess) e = dalloc(&p->pp_d, (size_t)std::max<int64_t>(npp, 1));
+  if (e == hipSuccess) e = dalloc(&p->pp_pos_d, (size_t)std::max<int64_t>(npp, 1));
+  if (e == hipSuccess) e = dalloc(&pp_tmp, (size_t)std::max<int64_t>(npp, 1));
   if (e == hipSuccess) e = dalloc(&pairs_d, pairs.size());
   if (e == hipSuccess && max_gen > 0) e = dalloc(&p->scratch_d, (size_t)max_gen);
   if (e == hipSuccess && max_flags > 0) e = dalloc(&p->flags_d, (size_t)max_flags);
@@ -746,10 +751,11 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     e = hipMemcpy(pairs_d, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = launch_pairdata(b->vsh_d, b->env_d, nv, pairs_d, (int)(pairs.size() / 2), p->pair_off_d,
-                        p->pair_ld_d, p->pp_d, nullptr);
+                        p->pair_ld_d, pp_tmp, p->pp_d, p->pp_pos_d, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   timer.mark("device upload, pair data");
   dfree(pairs_d);
+  dfree(pp_tmp);
   if (e != hipSuccess) {
     set_err(std::string("mcint_plan_create: ") + hipGetErrorString(e));
     plan_free(p);
@@ -786,6 +792,7 @@ static BasisView plan_view(const mcint_plan* p) {
   B.pair_off = p->pair_off_d;
   B.pair_ld = p->pair_ld_d;
   B.pp = p->pp_d;
+  B.pp_pos = p->pp_pos_d;
   B.nv = b->nv;
   B.expcutoff_4c = b->expc4;
   B.expcutoff_3c = b->expc3;

@@ -19,10 +19,15 @@ namespace mcint {
 // Pair data: one block per ordered virtual-shell pair, one lane per
 // primitive pair.  Restates CINTset_pairdata (src/optimizer.c:288-342).
 // ---------------------------------------------------------------------
+// One workgroup per shell pair.  The records are staged in 'tmp' and then
+// stored in ascending order of cce0 (ties by primitive index), so that the
+// primitive quartets passing the screening tests of one pair against a
+// fixed partner form a prefix of the list.
 __global__ void pairdata_kernel(const VShell* __restrict__ vsh, const double* __restrict__ env,
                                 int nv, const int* __restrict__ pairs,
                                 const int64_t* __restrict__ pair_off, double* __restrict__ pair_ld,
-                                PairPrim* __restrict__ pp) {
+                                PairPrim* __restrict__ tmp, PairPrim* __restrict__ pp,
+                                int* __restrict__ pos) {
   const int ia = pairs[2 * blockIdx.x];
   const int ib = pairs[2 * blockIdx.x + 1];
   const VShell a = vsh[ia];
@@ -55,16 +60,29 @@ __global__ void pairdata_kernel(const VShell* __restrict__ vsh, const double* __
     q.cce0 = e - log_rr0 - log(mca) - log(mcb);
     q.ai = ai;
     q.aj = aj;
-    pp[base + p] = q;
+    tmp[base + p] = q;
+  }
+  __syncthreads();
+  // rank sort; a NaN key (never produced by finite inputs) sorts last
+  auto key = [](double c) { return c == c ? c : __builtin_huge_val(); };
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const double c = key(tmp[base + p].cce0);
+    int rank = 0;
+    for (int m = 0; m < n; ++m) {
+      const double cm = key(tmp[base + m].cce0);
+      rank += (cm < c) || (cm == c && m < p);
+    }
+    pp[base + rank] = tmp[base + p];
+    pos[base + p] = rank;
   }
 }
 
 hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const int* pairs,
-                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* pp,
-                           hipStream_t stream) {
+                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* tmp,
+                           PairPrim* pp, int* pos, hipStream_t stream) {
   if (npairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(pairdata_kernel, dim3(npairs), dim3(64), 0, stream, vsh, env, nv, pairs,
-                     pair_off, pair_ld, pp);
+                     pair_off, pair_ld, tmp, pp, pos);
   return hipGetLastError();
 }
 
@@ -185,8 +203,14 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
 
   const int nij = s0.nprim * s1.nprim;
   const int nkl = three ? s2.nprim : s2.nprim * s3.nprim;
-  const PairPrim* pij = B.pp + B.pair_off[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
-  const PairPrim* pkl = three ? nullptr : B.pp + B.pair_off[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
+  // records visited in libcint's order (ip fastest), so that the sums over
+  // primitives round as libcint's do
+  const int64_t oij = B.pair_off[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
+  const int64_t okl = three ? 0 : B.pair_off[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
+  const PairPrim* pij = B.pp + oij;
+  const PairPrim* pkl = three ? nullptr : B.pp + okl;
+  const int* pos_ij = B.pp_pos + oij;
+  const int* pos_kl = B.pp_pos + okl;
   const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
   const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
@@ -248,7 +272,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
     for (int p = 0; p < nij * nkl; ++p) {
       const int ij = p % nij;
       const int kl = p / nij;
-      const PairPrim P = pij[ij];
+      const PairPrim P = pij[pos_ij[ij]];
       double cce_kl, akl, qx, qy, qz, kkl;
       if (three) {
         cce_kl = 0;
@@ -256,7 +280,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         qx = Cx; qy = Cy; qz = Cz;
         kkl = ck[kl];
       } else {
-        const PairPrim Q = pkl[kl];
+        const PairPrim Q = pkl[pos_kl[kl]];
         cce_kl = Q.cce0 - dkl_term;
         akl = Q.aij;
         qx = Q.px; qy = Q.py; qz = Q.pz;

@@ -141,14 +141,41 @@ __device__ __forceinline__ void vrr2d(R (&V)[E + 1][F + 1], R v0, R c00, R c0p, 
   });
 }
 
+// Length of the prefix of [0, n) on which the monotone predicate ok holds.
+template <class F>
+__device__ __forceinline__ int prefix_len(int n, F ok) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ok(mid)) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// LDS writes of some lanes of a wave made visible to the others.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row starts of the screening staircase: STAIR_ROWS_PER_LANE * T + 1 ints
+// per team of T lanes (teams with more rows test every quartet instead).
+constexpr int STAIR_ROWS_PER_LANE = 8;
+constexpr int STAIR_LDS_INTS = STAIR_ROWS_PER_LANE * 256 + 256;
+__device__ __forceinline__ int* stair_rows(int* s, int T) {
+  return s + (threadIdx.x / T) * (STAIR_ROWS_PER_LANE * T + 1);
+}
+
 // Primitive loop of one team: accumulates the [e0|f0] components with flat
 // index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
 // is the team's pair-emptiness verdict, 'any' whether this lane saw a
 // primitive quartet that passed libcint's two screening tests.
 template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
-                                           const double* __restrict__ rtab, R (&acc)[K1 - K0],
-                                           int& any, int& live) {
+                                           const double* __restrict__ rtab, int* __restrict__ rows,
+                                           R (&acc)[K1 - K0], int& any, int& live) {
   using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
   const VShell& sa = Bv.vsh[tk.sh[0]];
@@ -198,25 +225,92 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     }
   }
 
-  // pair emptiness: libcint returns early when a pair has no primitive below
-  // the cutoff (src/cint2e.c:96-109); the three-centre ket has no pair
-  int live_ij = 0, live_kl = THREE ? 1 : 0;
-  for (int p = tl; p < nij; p += T) live_ij |= (pij[p].cce0 - ldij) < Ecut;
-  if constexpr (!THREE)
-    for (int p = tl; p < nkl; p += T) live_kl |= (pkl[p].cce0 - ld_kl) < Ecut;
-  for (int off = 1; off < T; off <<= 1) {
-    live_ij |= __shfl_xor(live_ij, off);
-    live_kl |= __shfl_xor(live_kl, off);
-  }
-  live = live_ij && live_kl;
+  // Screening (src/cint2e.c:96-109 and the primitive loops of
+  // src/cint2e.c:120-300): a primitive quartet is computed when
+  //   cce_kl <= Eloop, cce_ij <= Eloop - cce_kl, cce_ij < Ecut, cce_kl < Ecut
+  // with cce = cce0 - distance term.  Each pair's records are stored in
+  // ascending cce0, so for a fixed row (one record of the shorter list) the
+  // passing records of the other list are a prefix, and the prefix lengths
+  // do not grow from row to row: the passing set is a staircase, and the
+  // team walks it without visiting the quartets that fail.
+  auto ok = [&](double cij, double ckl) {
+    return ckl < Ecut && cij < Ecut && !(ckl > Eloop) && !(cij > Eloop - ckl);
+  };
+  // pair emptiness (the first record is the pair's smallest); the
+  // three-centre ket has no pair
+  const double cij0 = pij[0].cce0 - ldij;
+  const double ckl0 = THREE ? 0.0 : pkl[0].cce0 - ld_kl;
+  live = cij0 < Ecut && (THREE || ckl0 < Ecut);
 
   sfor<K1 - K0>([&](auto I) { acc[decltype(I)::value] = R(0); });
-  any = 0;
+  // a quartet counts as non-zero when some primitive quartet passes the
+  // first two tests; for sorted lists that is the first one (short range
+  // decides per primitive quartet, below)
+  any = SR ? 0 : (live && !(ckl0 > Eloop) && !(cij0 > Eloop - ckl0));
   if (!live) return;
-  const int np = nij * nkl;
-  int kl = tl / nij;
-  int ij = tl - kl * nij;
-  for (int p = tl; p < np; p += T) {
+
+  // rows: the shorter list (the aux primitives for three centres); row r
+  // holds the columns [rb, re) of the flat index f
+  bool by_kl = true, stair = false;
+  int nrow, ncol_u = 0, S;
+  if constexpr (THREE) {
+    nrow = nkl;
+    ncol_u = prefix_len(nij, [&](int c) { return ok(pij[c].cce0 - ldij, 0.0); });
+    S = ncol_u * nkl;
+  } else {
+    by_kl = nkl <= nij;
+    nrow = by_kl ? nkl : nij;
+    const int ncol = by_kl ? nij : nkl;
+    if (nrow > STAIR_ROWS_PER_LANE * T || nrow * ncol <= T) {
+      // row starts do not fit the team's LDS, or one pass of the team covers
+      // every quartet anyway: every quartet, tested
+      ncol_u = ncol;
+      S = nrow * ncol;
+    } else {
+      stair = true;
+      // lane tl takes rows [r0, r1); lengths first, then starts in place
+      const int rpl = (nrow + T - 1) / T;
+      const int r0 = min(tl * rpl, nrow), r1 = min(r0 + rpl, nrow);
+      int hi = ncol, sum = 0;
+      for (int r = r0; r < r1; ++r) {
+        int len;
+        if (by_kl) {
+          const double ck = pkl[r].cce0 - ld_kl;
+          len = prefix_len(hi, [&](int c) { return ok(pij[c].cce0 - ldij, ck); });
+        } else {
+          const double ci = pij[r].cce0 - ldij;
+          len = prefix_len(hi, [&](int c) { return ok(ci, pkl[c].cce0 - ld_kl); });
+        }
+        hi = len;  // lengths do not grow with r
+        rows[r] = len;
+        sum += len;
+      }
+      int incl = sum;
+      for (int off = 1; off < T; off <<= 1) {
+        const int v = __shfl_up(incl, off, T);
+        if (tl >= off) incl += v;
+      }
+      S = __shfl(incl, T - 1, T);
+      int start = incl - sum;
+      for (int r = r0; r < r1; ++r) {
+        const int len = rows[r];
+        rows[r] = start;
+        start += len;
+      }
+      if (tl == 0) rows[nrow] = S;
+      wave_lds_sync();
+    }
+  }
+
+  int r = 0, rb = 0, re = stair ? rows[1] : ncol_u;
+  for (int f = tl; f < S; f += T) {
+    while (re <= f) {
+      ++r;
+      rb = re;
+      re = stair ? rows[r + 1] : re + ncol_u;
+    }
+    const int ij = by_kl ? f - rb : r;
+    const int kl = by_kl ? r : f - rb;
     PairPrim Q;
     if constexpr (THREE) {
       // ket "pair" of aux primitive kl and the exponent-0 s function
@@ -230,16 +324,9 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       Q = pkl[kl];
     }
     const PairPrim P = pij[ij];
-    // advance (ij fastest) before any 'continue'
-    ij += T;
-    while (ij >= nij) {
-      ij -= nij;
-      ++kl;
-    }
     const double cce_kl = Q.cce0 - ld_kl;
-    if (cce_kl > Eloop) continue;
     const double cce_ij = P.cce0 - ldij;
-    if (cce_ij > Eloop - cce_kl) continue;
+    if (!ok(cce_ij, cce_kl)) continue;  // only the untested fallback skips here
 
     const double aij = P.aij, akl = Q.aij;
     const double a1 = aij * akl;
@@ -250,14 +337,10 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     if constexpr (SR) {
       // pairs beyond the plain cutoff carry no data (centre at 1e18), and
       // negligible erfc tails are dropped (src/g2e.c:4449-4453, EXPCUTOFF_SR)
-      if (cce_ij >= Ecut) continue;
       theta = w2 / (w2 + a0);
       const double tx = theta * a0 * rr;
       if (tx > Eloop - cce_kl - cce_ij || tx > 40.0) continue;
       any = 1;
-    } else {
-      any = 1;
-      if (cce_ij >= Ecut || cce_kl >= Ecut) continue;  // libcint stores exp factor 0
     }
     const double fac1 = sqrt(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
     const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
@@ -338,6 +421,7 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int NE = C::NE, NF = C::NF;
   __shared__ double s_rys[RysLds<C::NR>::N];
+  __shared__ int s_rows[DER == 2 || DER == 3 ? 1 : STAIR_LDS_INTS];
   const double* rtab = rys_stage_table<C::NR>(s_rys);
   const int T = A.team;
   const int tid = threadIdx.x;
@@ -353,7 +437,8 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
   R acc[NE * NF];
   int any, live;
-  accumulate<LA, LB, LC, LD, 0, NE * NF, R, DER>(Bv, tk, T, tl, rtab, acc, any, live);
+  accumulate<LA, LB, LC, LD, 0, NE * NF, R, DER>(Bv, tk, T, tl, rtab, stair_rows(s_rows, T), acc,
+                                                 any, live);
 
   // team sum
   for (int off = 1; off < T; off <<= 1) {
@@ -470,12 +555,21 @@ __global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs 
 // cooperatively in LDS.
 // ---------------------------------------------------------------------
 constexpr int CHUNK_TARGET = 96;
+// Plain classes of at most three roots (la+lb+lc+ld <= 5) split finer: their
+// roots and recurrences are cheap to recompute per chunk, and a block of
+// <= 48 accumulators leaves room for two waves per SIMD (measured on the
+// c1 workload: (pp|pp) 48 -> 38 ms, (dp|pp) 43 -> 33 ms, (fp|p s) 31 -> 25 ms);
+// four-root classes lose more to the recomputation than they gain.
+constexpr int chunk_target(int la, int lb, int lc, int ld, int der) {
+  return (der == 0 && la + lb + lc + ld <= 5) ? 48 : CHUNK_TARGET;
+}
 
 template <int LA, int LB, int LC, int LD, int DER = 0>
 struct ChunkCfg {
   using C = FastClass<LA, LB, LC, LD, DER>;
   static constexpr int NEF = C::NE * C::NF;
-  static constexpr int NCH = (NEF + CHUNK_TARGET - 1) / CHUNK_TARGET;
+  static constexpr int TGT = chunk_target(LA, LB, LC, LD, DER);
+  static constexpr int NCH = (NEF + TGT - 1) / TGT;
   static constexpr int W = (NEF + NCH - 1) / NCH;
   static constexpr int k0(int c) { return c * W; }
   static constexpr int k1(int c) { return (c + 1) * W < NEF ? (c + 1) * W : NEF; }
@@ -486,6 +580,7 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
   __shared__ double s_rys[RysLds<G::C::NR>::N];
+  __shared__ int s_rows[DER == 2 || DER == 3 ? 1 : STAIR_LDS_INTS];
   const double* rtab = rys_stage_table<G::C::NR>(s_rys);
   const int T = A.team;
   const int tid = threadIdx.x;
@@ -495,7 +590,8 @@ __global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs
   const Task tk = A.tasks[t];
   R acc[K1 - K0];
   int any, live;
-  accumulate<LA, LB, LC, LD, K0, K1, R, DER>(Bv, tk, T, tl, rtab, acc, any, live);
+  accumulate<LA, LB, LC, LD, K0, K1, R, DER>(Bv, tk, T, tl, rtab, stair_rows(s_rows, T), acc, any,
+                                             live);
   for (int off = 1; off < T; off <<= 1) {
     sfor<K1 - K0>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -929,12 +1025,12 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
 // (fast_<g>.hip).
 #define MCINT_FAST_GROUP0(X) X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(2, 0, 0, 0) X(3, 0, 0, 0)
 #define MCINT_FAST_GROUP1(X) X(1, 1, 1, 0) X(2, 0, 1, 0) X(2, 1, 0, 0) X(2, 2, 0, 0) X(3, 1, 0, 0)
-#define MCINT_FAST_GROUP2(X) X(1, 1, 1, 1) X(3, 0, 1, 0) X(3, 2, 0, 0)
-#define MCINT_FAST_GROUP3(X) X(2, 0, 1, 1) X(2, 1, 1, 0)
+#define MCINT_FAST_GROUP2(X) X(3, 0, 1, 0) X(3, 2, 0, 0)
+#define MCINT_FAST_GROUP3(X) X(2, 1, 1, 0)
 #define MCINT_FAST_GROUP4(X) X(2, 0, 2, 0)
-#define MCINT_FAST_GROUP5(X) X(2, 1, 2, 0) X(3, 0, 1, 1)
-#define MCINT_FAST_GROUP6(X) X(2, 2, 1, 0) X(3, 0, 2, 0) X(3, 1, 1, 0)
-#define MCINT_FAST_GROUP7(X) X(3, 0, 3, 0)
+#define MCINT_FAST_GROUP5(X) X(3, 0, 3, 0)
+#define MCINT_FAST_GROUP6(X) X(2, 1, 2, 0)
+#define MCINT_FAST_GROUP7(X)
 #define MCINT_FAST_GROUP8(X) X(3, 3, 0, 0) X(3, 2, 1, 0)
 #define MCINT_FAST_CLASSES(X)                                                         \
   MCINT_FAST_GROUP0(X) MCINT_FAST_GROUP1(X) MCINT_FAST_GROUP2(X) MCINT_FAST_GROUP3(X) \
@@ -966,13 +1062,17 @@ hipError_t launch_chunked_class(const BasisView& B, const BucketArgs& A, hipStre
 // accumulator blocks of 144..160 doubles: two chunks rather than a spilling
 // single kernel
 #define MCINT_CHUNK_GROUP18(X) X(2, 1, 1, 1) X(3, 1, 2, 0)
-#define MCINT_CHUNK_GROUP19(X) X(3, 0, 2, 1)
+#define MCINT_CHUNK_GROUP19(X) X(3, 0, 2, 1) X(1, 1, 1, 1)
+// 54..90 accumulators, at most three roots: two chunks beat one kernel
+#define MCINT_CHUNK_GROUP20(X) X(2, 0, 1, 1) X(3, 0, 1, 1)
+#define MCINT_CHUNK_GROUP21(X) X(2, 2, 1, 0) X(3, 0, 2, 0) X(3, 1, 1, 0)
 #define MCINT_CHUNK_CLASSES(X)                                                              \
   MCINT_CHUNK_GROUP0(X) MCINT_CHUNK_GROUP1(X) MCINT_CHUNK_GROUP2(X) MCINT_CHUNK_GROUP3(X)   \
   MCINT_CHUNK_GROUP4(X) MCINT_CHUNK_GROUP5(X) MCINT_CHUNK_GROUP6(X) MCINT_CHUNK_GROUP7(X)   \
   MCINT_CHUNK_GROUP8(X) MCINT_CHUNK_GROUP9(X) MCINT_CHUNK_GROUP10(X) MCINT_CHUNK_GROUP11(X) \
   MCINT_CHUNK_GROUP12(X) MCINT_CHUNK_GROUP13(X) MCINT_CHUNK_GROUP14(X) MCINT_CHUNK_GROUP15(X) \
-  MCINT_CHUNK_GROUP16(X) MCINT_CHUNK_GROUP17(X) MCINT_CHUNK_GROUP18(X) MCINT_CHUNK_GROUP19(X)
+  MCINT_CHUNK_GROUP16(X) MCINT_CHUNK_GROUP17(X) MCINT_CHUNK_GROUP18(X) MCINT_CHUNK_GROUP19(X) \
+  MCINT_CHUNK_GROUP20(X) MCINT_CHUNK_GROUP21(X)
 
 // i-derivative (int2e_ip1) classes on the chunked path: shells up to d,
 // canonical LC >= LD (the derivative shell i stays first), one translation

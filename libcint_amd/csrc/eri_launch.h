@@ -74,9 +74,12 @@ __host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int l
   return g;
 }
 
+// Primitive-pair records of the listed shell pairs, each pair's list in
+// ascending order of cce0, and pos[k] = position in pp of the k-th record in
+// libcint's order; tmp is a staging buffer of the size of pp.
 hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const int* pairs,
-                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* pp,
-                           hipStream_t stream);
+                           int npairs, const int64_t* pair_off, double* pair_ld, PairPrim* tmp,
+                           PairPrim* pp, int* pos, hipStream_t stream);
 
 hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, double* w,
                             hipStream_t stream);

@@ -61,7 +61,8 @@ struct BasisView {
   const double* env;
   const int64_t* pair_off;   // [nv * nv], offset of pair (a,b) in pp[]
   const double* pair_ld;     // [nv * nv], log(|A-B| + 1)
-  const PairPrim* pp;
+  const PairPrim* pp;        // per pair, in ascending cce0
+  const int* pp_pos;         // [record in libcint's order (ip fastest)] -> position in pp
   int nv;
   double expcutoff_4c;  // libcint's cutoff for 4c integrals
   double expcutoff_3c;

@@ -356,3 +356,102 @@ def test_c4_3c2e_fp32_tier(mc):
     p64.close()
     p32.close()
     db.close()
+
+
+def _ip1_translation_residual(batch, q, bas):
+    """Per quartet, max |d/dA + d/dB + d/dC + d/dD| of int2e_ip1_sph, the four
+    derivatives taken from the ip1 blocks of (ij|kl), (ji|kl), (kl|ij),
+    (lk|ij); zero up to rounding by translational invariance."""
+    perm = np.concatenate([q[:, [0, 1, 2, 3]], q[:, [1, 0, 2, 3]], q[:, [2, 3, 0, 1]],
+                           q[:, [3, 2, 0, 1]]])
+    out, off, _ = batch("int2e_ip1_sph", perm)
+    d = 2 * bas[:, 1] + 1
+    n = q.shape[0]
+    res = np.zeros(n)
+    for t in range(n):
+        i, j, k, l = (int(d[s]) for s in q[t])
+        blk = [out[off[t + m * n]:off[t + m * n + 1]] for m in range(4)]
+        tot = (blk[0].reshape(3, l, k, j, i)
+               + blk[1].reshape(3, l, k, i, j).transpose(0, 1, 2, 4, 3)
+               + blk[2].reshape(3, j, i, l, k).transpose(0, 3, 4, 1, 2)
+               + blk[3].reshape(3, j, i, k, l).transpose(0, 4, 3, 1, 2))
+        res[t] = np.abs(tot).max()
+    return res
+
+
+def _cmp_ip1(mc, orc, m, q, got, ref, off, what):
+    """ip1 parity: every element within the bar, except in quartets where the
+    reference's own result is off by more than that (its translational
+    invariance residual exceeds the bar); there the GPU result must satisfy
+    invariance much better than the reference and differ from it by no more
+    than the reference's residual."""
+    err = np.abs(got - ref)
+    bad = err > ATOL + RTOL * np.abs(ref)
+    if not bad.any():
+        return
+    qi = np.unique(np.searchsorted(off, np.nonzero(bad)[0], side="right") - 1)
+    qb = q[qi]
+    r_ref = _ip1_translation_residual(
+        lambda n, qq: orc.batch(n, qq, orc.Basis(m.atm, m.bas, m.env)), qb, m.bas)
+    r_gpu = _ip1_translation_residual(
+        lambda n, qq: mc.batch(n, qq, mc.Basis(m.atm, m.bas, m.env)), qb, m.bas)
+    for t, n in enumerate(qi):
+        e = err[off[n]:off[n + 1]].max()
+        _say(what, "quartet", qb[t], "off by", e, "reference residual", r_ref[t],
+             "gpu residual", r_gpu[t])
+        assert r_ref[t] > ATOL and e <= 2 * r_ref[t] + ATOL and r_gpu[t] <= 0.1 * r_ref[t], \
+            (what, qb[t], e, r_ref[t], r_gpu[t])
+
+
+def test_ip1_reference_rounding_on_tight_primitives(mc, orc):
+    """(p f|d f) with p primitives up to exponent 1968 and |A-B| = 4 bohr:
+    libcint's transfer of the 2a-weighted raised integrals, done per
+    primitive quartet, loses about 1e-10 relative on the derivative block of
+    the p shell (its translational-invariance residual is 1.5e-11), while the
+    GPU path transfers after contraction and keeps the residual below 1e-12.
+    The two differ by the reference's own rounding error."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c1")
+    q = np.array([[44, 79, 48, 79]], np.int32)
+    orc.set_root_mode(orc.ROOTS_EXACT)
+    B = orc.Basis(m.atm, m.bas, m.env)
+    r_ref = _ip1_translation_residual(lambda n, qq: orc.batch(n, qq, B), q, m.bas)[0]
+    r_gpu = _ip1_translation_residual(
+        lambda n, qq: mc.batch(n, qq, mc.Basis(m.atm, m.bas, m.env)), q, m.bas)[0]
+    got, off, _ = mc.batch("int2e_ip1_sph", q, mc.Basis(m.atm, m.bas, m.env))
+    ref, _, _ = orc.batch("int2e_ip1_sph", q, B)
+    _say("tight-primitive ip1: reference residual", r_ref, "gpu residual", r_gpu,
+         "difference", np.abs(got - ref).max())
+    assert r_ref > 1e-12 and r_gpu < 1e-12
+    assert np.abs(got - ref).max() <= 2 * r_ref
+
+
+@pytest.mark.parametrize("ppl", [1, 4096])
+def test_team_shapes_and_screening_walk(mc, orc, ppl):
+    """The same contracted quartets computed by teams of 64 lanes (1 primitive
+    quartet per lane: the screening staircase in LDS) and by single lanes
+    (4096 per lane: rows beyond the LDS row budget, every primitive quartet
+    tested) agree with the oracle, values and non-zero flags."""
+    from libcint_amd import autotune
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c1")
+    q = _sample_quartets(m.nbas, 1500, seed=77)
+    _say("c1 sample, primitive quartets per lane", ppl)
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        for name in ("int2e_sph", "int2e_ip1_sph"):
+            autotune.reset(name)
+        mc.tune(mc.TUNE_PRIMS_PER_LANE, ppl)
+        for name in ("int2e_sph", "int2e_ip1_sph"):
+            qq = q if name == "int2e_sph" else q[:400]
+            got, off, nz = mc.batch(name, qq, mc.Basis(m.atm, m.bas, m.env))
+            ref, _, nz_r = orc.batch(name, qq, orc.Basis(m.atm, m.bas, m.env))
+            if name == "int2e_sph":
+                _cmp(got, ref, f"ppl {ppl} {name}")
+            else:
+                _cmp_ip1(mc, orc, m, qq, got, ref, off, f"ppl {ppl} {name}")
+            np.testing.assert_array_equal(nz, nz_r)
+    finally:
+        mc.tune(mc.TUNE_PRIMS_PER_LANE, 32)
+        autotune.apply(autotune.load())
+        orc.set_root_mode(orc.ROOTS_EXACT)
