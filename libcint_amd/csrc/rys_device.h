@@ -45,11 +45,18 @@ __device__ __forceinline__ void rys_root_one(int n, int r, double x, double& u, 
   }
 }
 
-// Root tables small enough to stage in LDS (n <= 4: at most 52.5 KB).
+// Root tables small enough to stage in LDS (n <= 4: at most 54 KB).  In
+// LDS each interval's block of RYS_NCOEF * 2n coefficients is padded by two
+// doubles (keeping 16-byte alignment): lanes reading different intervals
+// then start on different LDS banks (an unpadded block is a multiple of 8
+// banks long).
 constexpr int RYS_LDS_NMAX = 4;
 template <int NR>
 struct RysLds {
-  static constexpr int N = NR <= RYS_LDS_NMAX ? RYS_XMAX(NR) * RYS_NCOEF * 2 * NR : 1;
+  static constexpr bool IN_LDS = NR <= RYS_LDS_NMAX;
+  static constexpr int BLOCK = RYS_NCOEF * 2 * NR;            // one interval, global layout
+  static constexpr int STRIDE = IN_LDS ? BLOCK + 2 : BLOCK;   // one interval, as evaluated
+  static constexpr int N = IN_LDS ? RYS_XMAX(NR) * STRIDE : 1;
 };
 
 // Copies the n = NR table into s (all threads of the block; ends with a
@@ -57,9 +64,13 @@ struct RysLds {
 // when the table does not fit.
 template <int NR>
 __device__ __forceinline__ const double* rys_stage_table(double* s) {
-  if constexpr (NR <= RYS_LDS_NMAX) {
+  using L = RysLds<NR>;
+  if constexpr (L::IN_LDS) {
     const double* g = rys_coef + rys_table_offset[NR];
-    for (int i = threadIdx.x; i < RysLds<NR>::N; i += blockDim.x) s[i] = g[i];
+    for (int i = threadIdx.x; i < RYS_XMAX(NR) * L::BLOCK; i += blockDim.x) {
+      const int iv = i / L::BLOCK;
+      s[i + 2 * iv] = g[i];  // = iv * STRIDE + (i - iv * BLOCK)
+    }
     __syncthreads();
     return s;
   } else {
@@ -84,7 +95,7 @@ __device__ __forceinline__ void rys_roots(double x, double* u, double* w,
   } else {
     const int it = (int)x;
     const double z = 2.0 * (x - (double)it) - 1.0;
-    const double* c = tab + it * (RYS_NCOEF * 2 * NR);
+    const double* c = tab + it * RysLds<NR>::STRIDE;
     double acc[2 * NR];
 #pragma unroll
     for (int f = 0; f < 2 * NR; ++f) acc[f] = c[(RYS_NCOEF - 1) * 2 * NR + f];
