@@ -416,8 +416,31 @@ __device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& t
   st3 = so[(perm >> 6) & 3];
 }
 
+// Waves per SIMD the register allocator is asked to keep room for (1 = no
+// request), per plain class (chunk = -1: the single-chunk kernel).  Kept
+// where measured faster on the c1 workload: (pp|ps) 172 -> 168 VGPRs, three
+// waves, 37.2 -> 30.7 ms; (fs|ds) chunks 6.4 -> 5.7 ms.  Requests that cost
+// more than ~30 bytes of scratch spill per lane lost ((ps|ps), (ps|ss),
+// (ds|ps), (fs|ps), (pp|pp), (fp|ps)).
+constexpr int min_waves(int la, int lb, int lc, int ld, int der, int chunk) {
+  if (der != 0) return 1;
+  const int c = ((la * 4 + lb) * 4 + lc) * 4 + ld;
+  if (chunk < 0) {
+    switch (c) {
+      case ((1 * 4 + 1) * 4 + 1) * 4 + 0: return 3;
+      default: return 1;
+    }
+  }
+  switch (c) {
+    case ((3 * 4 + 0) * 4 + 2) * 4 + 0: return 3;
+    default: return 1;
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0>
-__global__ void __launch_bounds__(256) eri_fast_kernel(BasisView Bv, BucketArgs A) {
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(min_waves(LA, LB, LC, LD, DER, -1))))
+eri_fast_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int NE = C::NE, NF = C::NF;
   __shared__ double s_rys[RysLds<C::NR>::N];
@@ -576,7 +599,9 @@ struct ChunkCfg {
 };
 
 template <int LA, int LB, int LC, int LD, int CH, int DER = 0, typename R = double>
-__global__ void __launch_bounds__(256) eri_chunk_kernel(BasisView Bv, BucketArgs A) {
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(min_waves(LA, LB, LC, LD, DER, CH))))
+eri_chunk_kernel(BasisView Bv, BucketArgs A) {
   using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
   __shared__ double s_rys[RysLds<G::C::NR>::N];
