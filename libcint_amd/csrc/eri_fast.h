@@ -638,6 +638,22 @@ eri_chunk_kernel(BasisView Bv, BucketArgs A) {
   }
 }
 
+// (b, t) pairs of the transfer (a,b| = sum_t C(b,t) (A-B)^(b-t) [a+t| over
+// the Cartesian b of shell l, and nonzero Cartesian->spherical coefficients
+// of shell l (cost estimates for the epilogue layout).
+constexpr int xfer_terms(int l) {
+  int n = 0;
+  for (int i = 0; i <= l; ++i)
+    for (int j = 0; j <= l - i; ++j) n += (i + 1) * (j + 1) * (l - i - j + 1);
+  return n;
+}
+constexpr int nnz_c2s(int l) {
+  int n = 0;
+  for (int o = 0; o < 2 * l + 1; ++o)
+    for (int c = 0; c < (l + 1) * (l + 2) / 2; ++c) n += c2sc(l, o, c) != 0.0;
+  return n;
+}
+
 template <int LA, int LB, int LC, int LD>
 struct EpiCfg {
   using C = FastClass<LA, LB, LC, LD>;
@@ -646,10 +662,36 @@ struct EpiCfg {
   // 48 KB; then the bra transfer reads it from the scratch block
   static constexpr bool XG = NEF + C::NAo * C::NBo * C::NF > 6144;
   static constexpr int PER = (XG ? 0 : NEF) + C::NAo * C::NBo * C::NF;  // LDS doubles per task
-  static constexpr int WORK = C::NF > C::NAo * C::NBo ? C::NF : C::NAo * C::NBo;
-  static constexpr int TE = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
-  static constexpr int TPB0 = 256 / TE;
   static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
+  // Lanes per task: the bra pass gives each lane whole columns (NF of
+  // them), the ket pass whole rows (NAo * NBo).  By default TE covers the
+  // larger of the two.  For an (ff| bra (49+ rows, the costliest transfer)
+  // whose few columns leave most lanes idle in the bra pass, TE minimises
+  // the estimated wave instructions per task, (columns per lane * column
+  // cost + rows per lane * row cost) / tasks per wave, when that is 3.5x
+  // below the default's: (ff|ps) 8.0 -> 5.7 ms, (ff|ds) 4.9 -> 4.4 ms,
+  // (ff|fs) 4.8 -> 4.4 ms.  The estimate ignores occupancy; on other classes
+  // the smaller TE lost.
+  static constexpr int COL = C::NA * xfer_terms(LB) + nnz_c2s(LA) * nnz_c2s(LB);
+  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC) * nnz_c2s(LD) + C::NCo * C::NDo;
+  static constexpr int64_t cost(int te) {
+    const int tpb = (256 / te) < TPB_LDS ? 256 / te : TPB_LDS;
+    const int lanes = tpb * te < 64 ? tpb * te : 64;
+    const int64_t per_lane =
+        (int64_t)((C::NF + te - 1) / te) * COL + (int64_t)((C::NAo * C::NBo + te - 1) / te) * ROW;
+    return per_lane * 64 * te / lanes;  // x 64: tasks per wave = lanes / te
+  }
+  static constexpr int pick_te() {
+    int best = 64;
+    for (int te = 32; te >= 1; te /= 2)
+      if (cost(te) < cost(best)) best = te;
+    return best;
+  }
+  static constexpr int WORK = C::NF > C::NAo * C::NBo ? C::NF : C::NAo * C::NBo;
+  static constexpr int TE_COVER = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
+  static constexpr int TE = C::NAo * C::NBo >= 49 && 2 * cost(TE_COVER) > 7 * cost(pick_te())
+                                ? pick_te() : TE_COVER;
+  static constexpr int TPB0 = 256 / TE;
   static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
   static constexpr int THREADS = TPB * TE;
   static_assert(PER <= 6144, "epilogue exceeds its LDS budget");
