@@ -326,7 +326,8 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     const PairPrim P = pij[ij];
     const double cce_kl = Q.cce0 - ld_kl;
     const double cce_ij = P.cce0 - ldij;
-    if (!ok(cce_ij, cce_kl)) continue;  // only the untested fallback skips here
+    // the staircase holds passing quartets only; the fallback tests each
+    if (!stair && !ok(cce_ij, cce_kl)) continue;
 
     const double aij = P.aij, akl = Q.aij;
     const double a1 = aij * akl;
@@ -337,12 +338,12 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
     if constexpr (SR) {
       // pairs beyond the plain cutoff carry no data (centre at 1e18), and
       // negligible erfc tails are dropped (src/g2e.c:4449-4453, EXPCUTOFF_SR)
-      theta = w2 / (w2 + a0);
+      theta = w2 * rcp_pos(w2 + a0);
       const double tx = theta * a0 * rr;
       if (tx > Eloop - cce_kl - cce_ij || tx > 40.0) continue;
       any = 1;
     }
-    const double fac1 = sqrt(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
+    const double fac1 = sqrt_pos(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * Q.kab;
     const R sai = R(2.0 * P.ai);  // weight of the raised set (DER = 1)
     const double pax = P.px - Ax, pay = P.py - Ay, paz = P.pz - Az;
     const double qcx = Q.px - Cx, qcy = Q.py - Cy, qcz = Q.pz - Cz;
@@ -379,7 +380,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       double u[2 * NR], w[2 * NR];
       rys_roots<NR>(a0 * rr, u, w, rtab);
       rys_roots<NR>(theta * a0 * rr, u + NR, w + NR, rtab);
-      const double mst = -sqrt(theta);
+      const double mst = -sqrt_pos(theta);
       sfor<NR>([&](auto RI) {
         constexpr int r = NR + decltype(RI)::value;
         const double ut = u[r] * theta;

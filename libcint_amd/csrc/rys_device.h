@@ -21,6 +21,23 @@ __device__ __forceinline__ double rcp_pos(double d) {
   return fma(r, e, r);
 }
 
+// 1/sqrt(x) and sqrt(x) for positive normal x: the hardware reciprocal
+// square root refined by two Newton-Raphson steps y <- y (3/2 - x y^2 / 2),
+// and for sqrt one more residual correction s <- s + (x - s^2) y / 2 (error
+// about 1 ulp; the library sqrt also rescales denormal and special inputs,
+// which cannot occur here).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return fma(y, fma(-h * y, y, 0.5), y);
+}
+__device__ __forceinline__ double sqrt_pos(double x) {
+  const double y = rsqrt_pos(x);
+  const double s = x * y;
+  return fma(fma(-s, s, x), 0.5 * y, s);
+}
+
 // Root/weight number f (0 <= f < 2n: roots first, then weights) of the
 // n-point rule at x < RYS_XMAX(n).
 __device__ __forceinline__ double rys_poly_eval(int n, int f, double x) {
@@ -85,11 +102,11 @@ template <int NR>
 __device__ __forceinline__ void rys_roots(double x, double* u, double* w,
                                           const double* __restrict__ tab) {
   if (x >= (double)RYS_XMAX(NR)) {
-    const double rsx = 1.0 / sqrt(x);
+    const double rsx = rsqrt_pos(x);
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const double rr = rys_lag_r[rys_lag_offset[NR] + i];
-      u[i] = rr / (x - rr);
+      u[i] = rr * rcp_pos(x - rr);
       w[i] = rys_lag_w[rys_lag_offset[NR] + i] * rsx;
     }
   } else {
