@@ -859,10 +859,42 @@ struct Ip1EpiCfg {
   static constexpr int ROWS = 3 * C::NAo * C::NBo;
   static constexpr int PER = NEF + ROWS * C::NF;  // doubles of LDS per task
   static constexpr int WORK = C::NF > ROWS ? C::NF : ROWS;
-  static constexpr int TE = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
-  static constexpr int TPB0 = 256 / TE;
+  static constexpr int TE_COVER = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
   static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
-  static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
+  // Lanes per task: the bra pass gives each lane whole columns (three
+  // directions each), the digest pass whole rows.  TE_COVER covers the
+  // larger of the two; a smaller TE is taken when it fills whole waves,
+  // keeps at least 8 waves per CU within the LDS, and halves the estimated
+  // wave instructions per task (columns per lane * column cost + rows per
+  // lane * row cost, over tasks per wave).
+  static constexpr int COL = 3 * (C::NA * 2 * xfer_terms(LB) + nnz_c2s(LA) * nnz_c2s(LB));
+  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC) * nnz_c2s(LD) + 4 * C::NCo * C::NDo;
+  static constexpr int tpb_for(int te) {
+    const int t = (256 / te) < TPB_LDS ? 256 / te : TPB_LDS;
+    const int q = te < 64 ? 64 / te : 1;  // tasks per full wave
+    return t >= q ? t - t % q : t;
+  }
+  static constexpr bool usable(int te) {
+    const int tpb = tpb_for(te);
+    const int threads = tpb * te;
+    if (threads % 64) return false;
+    const int blocks = 163840 / (tpb * PER * 8 + 1);
+    return (blocks < 8 ? blocks : 8) * (threads / 64) >= 8;
+  }
+  static constexpr int64_t cost(int te) {
+    const int64_t per_lane = (int64_t)((C::NF + te - 1) / te) * COL +
+                             (int64_t)((ROWS + te - 1) / te) * ROW;
+    const int lanes = tpb_for(te) * te < 64 ? tpb_for(te) * te : 64;
+    return per_lane * 64 * te / lanes;
+  }
+  static constexpr int pick_te() {
+    int best = TE_COVER;
+    for (int te = TE_COVER / 2; te >= 1; te /= 2)
+      if (usable(te) && 2 * cost(te) <= cost(TE_COVER) && cost(te) < cost(best)) best = te;
+    return best;
+  }
+  static constexpr int TE = pick_te();
+  static constexpr int TPB = TE == TE_COVER ? ((256 / TE) < TPB_LDS ? 256 / TE : TPB_LDS) : tpb_for(TE);
   static constexpr int THREADS = TPB * TE;
   static_assert(PER <= 6144, "i-derivative epilogue exceeds its LDS budget");
 };
