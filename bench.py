@@ -80,7 +80,8 @@ def cpu_baseline(basis_np, quartets, target_s=12.0, max_threads=16):
 
 def gradient_section(reps: int = 3) -> dict:
     """The derivative and J/K legs on the 20-atom C8H12 cluster (DZ basis
-    with d on H): plain FP64 int2e_sph over all unique quartets, the J/K
+    with d on H): plain FP64 int2e_sph and int2e_cart over all unique
+    quartets, the J/K
     build (ERIs + grouped digest, FP64 and with the FP32 tier), and the
     nuclear-gradient digest over the ip1 list (i, j, k >= l), FP64 and with
     the FP32 tier (Schwarz x max|D|^2 < 1e-7), each timed on device-resident
@@ -108,6 +109,12 @@ def gradient_section(reps: int = 3) -> dict:
     plan = mc.Plan(db, "int2e_sph", q)
     out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
     t_plain, _ = timed(lambda: plan.execute(out.data_ptr(), graph=False))
+    plan.close()
+    del out
+    plan = mc.Plan(db, "int2e_cart", q)
+    out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+    t_cart, _ = timed(lambda: plan.execute(out.data_ptr(), graph=False))
+    cart_eris = plan.out_size
     plan.close()
     del out
     jb = jk.JKBuilder(B)
@@ -139,6 +146,8 @@ def gradient_section(reps: int = 3) -> dict:
         "workload": f"c3: {m.name}, {m.natm} atoms, {m.nbas} shells, {m.nao_sph()} AOs; "
                     f"random density D = A A^T / N",
         "plain_int2e_sph_ms": t_plain * 1e3,
+        "plain_int2e_cart_ms": t_cart * 1e3,
+        "plain_int2e_cart_eris": int(cart_eris),
         "plain_unique_quartets": int(q.shape[0]),
         "ip1_gradient_ms": t_grad * 1e3,
         "ip1_gradient_quartets": int(nq_grad),

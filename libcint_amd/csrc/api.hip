@@ -511,8 +511,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       const int ppl_c = g_class_ppl[kind][class_index(la, lb, lc, ld)];
       const int team = mode ? pick_team(nprimq, ppl_c ? ppl_c : g_prims_per_lane) : 0;
       // FP32 recurrences where a single-chunk kernel or a derivative chunk
-      // kernel exists (plain operator only); other flagged quartets stay FP64
-      const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 &&
+      // kernel exists (plain operator, spherical output only); other flagged
+      // quartets stay FP64
+      const uint32_t lowp = (fp32_flags && fp32_flags[q] && b->omega == 0 && sph &&
                              (mode == 1 || (mode == 2 && kind == KIND_2E_IP1)))
                                 ? 1u
                                 : 0u;

@@ -105,6 +105,13 @@ struct FastClass {
   static constexpr bool row_scaled(int i) { return D1 && i < NEP; }
 };
 
+// Output functions of shell l: spherical (2l+1, real solid harmonics) or
+// Cartesian (the identity transform).
+constexpr int nout(int l, bool sph) { return sph ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
+constexpr double c2x(bool sph, int l, int o, int k) {
+  return sph ? c2sc(l, o, k) : (o == k ? 1.0 : 0.0);
+}
+
 // Compile-time loop: f(std::integral_constant<int, 0..N-1>) expanded in place,
 // so every index derived from the loop variable is a constant expression.
 template <typename F, int... Is>
@@ -398,14 +405,15 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
 
 // Output strides of the canonical axes: axis a holds the original axis
 // (perm >> 2a) & 3 of the packed Fortran block.
+template <bool SPH = true>
 __device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& tk, int perm,
                                               int64_t& st0, int64_t& st1, int64_t& st2,
                                               int64_t& st3) {
   int dorig[4];
-  dorig[perm & 3] = Bv.vsh[tk.sh[0]].dsph;
-  dorig[(perm >> 2) & 3] = Bv.vsh[tk.sh[1]].dsph;
-  dorig[(perm >> 4) & 3] = Bv.vsh[tk.sh[2]].dsph;
-  dorig[(perm >> 6) & 3] = Bv.vsh[tk.sh[3]].dsph;
+  dorig[perm & 3] = (SPH ? Bv.vsh[tk.sh[0]].dsph : Bv.vsh[tk.sh[0]].dcart);
+  dorig[(perm >> 2) & 3] = (SPH ? Bv.vsh[tk.sh[1]].dsph : Bv.vsh[tk.sh[1]].dcart);
+  dorig[(perm >> 4) & 3] = (SPH ? Bv.vsh[tk.sh[2]].dsph : Bv.vsh[tk.sh[2]].dcart);
+  dorig[(perm >> 6) & 3] = (SPH ? Bv.vsh[tk.sh[3]].dsph : Bv.vsh[tk.sh[3]].dcart);
   int64_t so[4];
   so[0] = 1;
   so[1] = dorig[0];
@@ -438,7 +446,7 @@ constexpr int min_waves(int la, int lb, int lc, int ld, int der, int chunk) {
   }
 }
 
-template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0>
+template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0, bool SPH = true>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(min_waves(LA, LB, LC, LD, DER, -1))))
 eri_fast_kernel(BasisView Bv, BucketArgs A) {
@@ -477,11 +485,11 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
 
   // ---- epilogue --------------------------------------------------------
   int64_t st0, st1, st2, st3;
-  canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
+  canon_strides<SPH>(Bv, tk, A.perm, st0, st1, st2, st3);
   double* __restrict__ out = A.out + tk.out;
 
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
-  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
   if (!any) {
     sfor<NAo * NBo * NCo * NDo>([&](auto K) {
       constexpr int k = decltype(K)::value;
@@ -529,7 +537,7 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
       R s = R(0);
       sfor<NA * NB>([&](auto AB) {
         constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
-        constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+        constexpr double cc = c2x(SPH, LA, ao, a) * c2x(SPH, LB, bo, b);
         if constexpr (cc != 0.0) s = fma(R(cc), Yc[a * NB + b], s);
       });
       Ys[(ao * NBo + bo) * NF + j] = s;
@@ -562,7 +570,7 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
         R s = R(0);
         sfor<NC * ND>([&](auto CD) {
           constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
-          constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+          constexpr double cc = c2x(SPH, LC, co, c) * c2x(SPH, LD, dd, d);
           if constexpr (cc != 0.0) s = fma(R(cc), Zc[c * ND + d], s);
         });
         out[ao * st0 + bo * st1 + co * st2 + dd * st3] = (double)s;
@@ -648,22 +656,26 @@ constexpr int xfer_terms(int l) {
     for (int j = 0; j <= l - i; ++j) n += (i + 1) * (j + 1) * (l - i - j + 1);
   return n;
 }
-constexpr int nnz_c2s(int l) {
+constexpr int nnz_c2s(int l, bool sph = true) {
   int n = 0;
-  for (int o = 0; o < 2 * l + 1; ++o)
-    for (int c = 0; c < (l + 1) * (l + 2) / 2; ++c) n += c2sc(l, o, c) != 0.0;
+  for (int o = 0; o < nout(l, sph); ++o)
+    for (int c = 0; c < (l + 1) * (l + 2) / 2; ++c) n += c2x(sph, l, o, c) != 0.0;
   return n;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 struct EpiCfg {
   using C = FastClass<LA, LB, LC, LD>;
+  static constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
   static constexpr int NEF = C::NE * C::NF;
-  // the [e0|f0] block is staged in LDS unless block + bra result exceed
-  // 48 KB; then the bra transfer reads it from the scratch block
-  static constexpr bool XG = NEF + C::NAo * C::NBo * C::NF > 6144;
-  static constexpr int PER = (XG ? 0 : NEF) + C::NAo * C::NBo * C::NF;  // LDS doubles per task
-  static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
+  // LDS budget in doubles: 48 KB, 80 KB for Cartesian output (whose bra
+  // result of (ff|ff) alone is 58 KB)
+  static constexpr int BUDGET = SPH ? 6144 : 10240;
+  // the [e0|f0] block is staged in LDS unless block + bra result exceed the
+  // budget; then the bra transfer reads it from the scratch block
+  static constexpr bool XG = NEF + NAo * NBo * C::NF > BUDGET;
+  static constexpr int PER = (XG ? 0 : NEF) + NAo * NBo * C::NF;  // LDS doubles per task
+  static constexpr int TPB_LDS = BUDGET / PER > 0 ? BUDGET / PER : 1;
   // Lanes per task: the bra pass gives each lane whole columns (NF of
   // them), the ket pass whole rows (NAo * NBo).  By default TE covers the
   // larger of the two.  For an (ff| bra (49+ rows, the costliest transfer)
@@ -673,13 +685,13 @@ struct EpiCfg {
   // below the default's: (ff|ps) 8.0 -> 5.7 ms, (ff|ds) 4.9 -> 4.4 ms,
   // (ff|fs) 4.8 -> 4.4 ms.  The estimate ignores occupancy; on other classes
   // the smaller TE lost.
-  static constexpr int COL = C::NA * xfer_terms(LB) + nnz_c2s(LA) * nnz_c2s(LB);
-  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC) * nnz_c2s(LD) + C::NCo * C::NDo;
+  static constexpr int COL = C::NA * xfer_terms(LB) + nnz_c2s(LA, SPH) * nnz_c2s(LB, SPH);
+  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC, SPH) * nnz_c2s(LD, SPH) + NCo * NDo;
   static constexpr int64_t cost(int te) {
     const int tpb = (256 / te) < TPB_LDS ? 256 / te : TPB_LDS;
     const int lanes = tpb * te < 64 ? tpb * te : 64;
     const int64_t per_lane =
-        (int64_t)((C::NF + te - 1) / te) * COL + (int64_t)((C::NAo * C::NBo + te - 1) / te) * ROW;
+        (int64_t)((C::NF + te - 1) / te) * COL + (int64_t)((NAo * NBo + te - 1) / te) * ROW;
     return per_lane * 64 * te / lanes;  // x 64: tasks per wave = lanes / te
   }
   static constexpr int pick_te() {
@@ -688,25 +700,25 @@ struct EpiCfg {
       if (cost(te) < cost(best)) best = te;
     return best;
   }
-  static constexpr int WORK = C::NF > C::NAo * C::NBo ? C::NF : C::NAo * C::NBo;
+  static constexpr int WORK = C::NF > NAo * NBo ? C::NF : NAo * NBo;
   static constexpr int TE_COVER = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
-  static constexpr int TE = C::NAo * C::NBo >= 49 && 2 * cost(TE_COVER) > 7 * cost(pick_te())
+  static constexpr int TE = NAo * NBo >= 49 && 2 * cost(TE_COVER) > 7 * cost(pick_te())
                                 ? pick_te() : TE_COVER;
   static constexpr int TPB0 = 256 / TE;
   static constexpr int TPB = TPB0 < TPB_LDS ? TPB0 : TPB_LDS;
   static constexpr int THREADS = TPB * TE;
-  static_assert(PER <= 6144, "epilogue exceeds its LDS budget");
+  static_assert(PER <= BUDGET, "epilogue exceeds its LDS budget");
 };
 
 // Bra transfer (a,b| = sum_t C(b,t) (A-B)^(b-t) [a+t| and the spherical
 // transform over a, b for the columns j = j0, j0+js, ... of the contracted
 // block X (NE x NF), written to Ys (NAo*NBo x NF).
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
                                         const double* __restrict__ X, double* __restrict__ Ys,
                                         int j0, int js) {
   using C = FastClass<LA, LB, LC, LD>;
-  constexpr int NF = C::NF, NA = C::NA, NB = C::NB, NAo = C::NAo, NBo = C::NBo;
+  constexpr int NF = C::NF, NA = C::NA, NB = C::NB, NAo = nout(LA, SPH), NBo = nout(LB, SPH);
   const VShell& sa = Bv.vsh[tk.sh[0]];
   const VShell& sb = Bv.vsh[tk.sh[1]];
   const double ab[3] = {sa.r[0] - sb.r[0], sa.r[1] - sb.r[1], sa.r[2] - sb.r[2]};
@@ -738,7 +750,7 @@ __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
       double s = 0.0;
       sfor<NA * NB>([&](auto AB) {
         constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
-        constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+        constexpr double cc = c2x(SPH, LA, ao, a) * c2x(SPH, LB, bo, b);
         if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
       });
       Ys[(ao * NBo + bo) * NF + j] = s;
@@ -748,16 +760,16 @@ __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
 
 // Ket transfer and spherical c, d of the rows r0, r0+rs, ... of Ys, then
 // the store with the task's output strides; any == 0 writes zeros.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 __device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& tk, int perm,
                                               const double* __restrict__ Ys,
                                               double* __restrict__ outbase, int any, int r0,
                                               int rs) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int NF = C::NF, NC = C::NC, ND = C::ND;
-  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
   int64_t st0, st1, st2, st3;
-  canon_strides(Bv, tk, perm, st0, st1, st2, st3);
+  canon_strides<SPH>(Bv, tk, perm, st0, st1, st2, st3);
   double* __restrict__ out = outbase + tk.out;
   if (!any) {
     for (int k = r0; k < NAo * NBo * NCo * NDo; k += rs) {
@@ -801,7 +813,7 @@ __device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& t
       double s = 0.0;
       sfor<NC * ND>([&](auto CD) {
         constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
-        constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+        constexpr double cc = c2x(SPH, LC, co, c) * c2x(SPH, LD, dd, d);
         if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
       });
       orow[co * st2 + dd * st3] = s;
@@ -809,10 +821,10 @@ __device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& t
   }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD>;
-  using P = EpiCfg<LA, LB, LC, LD>;
+  using P = EpiCfg<LA, LB, LC, LD, SPH>;
   constexpr int NEF = P::NEF, TE = P::TE;
   __shared__ double sm[P::TPB * P::PER];
   const int team = threadIdx.x / TE;
@@ -835,9 +847,9 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
     }
   }
   __syncthreads();
-  if (valid && any) epi_bra<LA, LB, LC, LD>(Bv, tk, X, Ys, tl, TE);
+  if (valid && any) epi_bra<LA, LB, LC, LD, SPH>(Bv, tk, X, Ys, tl, TE);
   __syncthreads();
-  if (valid) epi_ket_store<LA, LB, LC, LD>(Bv, tk, A.perm, Ys, A.out, any, tl, TE);
+  if (valid) epi_ket_store<LA, LB, LC, LD, SPH>(Bv, tk, A.perm, Ys, A.out, any, tl, TE);
 }
 
 // ---------------------------------------------------------------------
@@ -1103,7 +1115,17 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
   hipError_t e = hipSuccess;
   ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
-  if constexpr (DER != 1) {
+  if constexpr (DER == 0) {
+    if (A.sph) {
+      using P = EpiCfg<LA, LB, LC, LD>;
+      const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+    } else {
+      using P = EpiCfg<LA, LB, LC, LD, false>;
+      const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD, false>), egrid, dim3(P::THREADS), 0, stream, B, A);
+    }
+  } else if constexpr (DER != 1) {
     using P = EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
     hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
@@ -1236,7 +1258,9 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
     const int64_t n = A.t1 - A.t0;                                                         \
     const int64_t per = 256 / A.team;                                                      \
     const dim3 grid((unsigned)((n + per - 1) / per));                                      \
-    if (A.fp32)                                                                            \
+    if (!A.sph)                                                                            \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 0, false>), grid, dim3(256), 0, stream, B, A); \
+    else if (A.fp32)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float>), grid, dim3(256), 0, stream, B, A); \
     else                                                                                   \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double>), grid, dim3(256), 0, stream, B, A); \
@@ -1335,7 +1359,7 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
   MCINT_FAST3_CLASSES(MCINT_FAST3_DECLARE)                                                 \
   MCINT_CHUNK3_CLASSES(MCINT_CHUNK3_DECLARE)                                               \
   int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld) {                \
-    if (!sph) return 0;                                                                    \
+    if (!sph && kind != KIND_2E) return 0;                                                 \
     if (kind == KIND_2E_IP1) {                                                             \
       MCINT_IP1_CLASSES(MC_CHUNK_HAS)                                                      \
       return 0;                                                                            \
