@@ -446,7 +446,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       for (int lc = 0; lc <= LMAX; ++lc)
         for (int ld = 0; ld <= LMAX; ++ld)
           mode_tab[((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld] =
-              (uint8_t)fast_kernel_mode(kind, sph, la, lb, lc, ld);
+              (uint8_t)fast_kernel_mode(kind, sph, la, lb, lc, ld, b->omega);
   std::vector<int64_t> first_bad(nthr, INT64_MAX);
   std::vector<int64_t> nprim_thr(nthr, 0), ntask_thr(nthr, 0);
   const bool reg_ok = b->omega == 0 || (b->omega < 0 && kind == KIND_3C2E);

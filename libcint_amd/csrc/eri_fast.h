@@ -1115,7 +1115,7 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
   hipError_t e = hipSuccess;
   ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
-  if constexpr (DER == 0) {
+  if constexpr (DER == 0 || DER == 2) {
     if (A.sph) {
       using P = EpiCfg<LA, LB, LC, LD>;
       const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
@@ -1282,6 +1282,8 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
     const dim3 grid((unsigned)((n + per - 1) / per));                                      \
     if (B.omega < 0)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 3>), grid, dim3(256), 0, stream, B, A); \
+    else if (!A.sph)                                                                       \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 2, false>), grid, dim3(256), 0, stream, B, A); \
     else if (A.fp32)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float, 2>), grid, dim3(256), 0, stream, B, A); \
     else                                                                                   \
@@ -1358,8 +1360,8 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
   MCINT_IP1_CLASSES(MCINT_IP1_DECLARE)                                                     \
   MCINT_FAST3_CLASSES(MCINT_FAST3_DECLARE)                                                 \
   MCINT_CHUNK3_CLASSES(MCINT_CHUNK3_DECLARE)                                               \
-  int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld) {                \
-    if (!sph && kind != KIND_2E) return 0;                                                 \
+  int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld, double omega) {  \
+    if (!sph && (kind == KIND_2E_IP1 || omega < 0)) return 0;                             \
     if (kind == KIND_2E_IP1) {                                                             \
       MCINT_IP1_CLASSES(MC_CHUNK_HAS)                                                      \
       return 0;                                                                            \

@@ -168,11 +168,13 @@ def test_c4_3c2e_sample(mc, orc):
     _say("c4 int3c2e sample")
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
-        got, _, _ = mc.batch("int3c2e_sph", q, mc.Basis(m.atm, m.bas, m.env))
-        ref, _, _ = orc.batch("int3c2e_sph", q, orc.Basis(m.atm, m.bas, m.env))
+        for name in ("int3c2e_sph", "int3c2e_cart"):
+            got, _, nz = mc.batch(name, q, mc.Basis(m.atm, m.bas, m.env))
+            ref, _, nz_r = orc.batch(name, q, orc.Basis(m.atm, m.bas, m.env))
+            _cmp(got, ref, f"c4 {name}")
+            np.testing.assert_array_equal(nz, nz_r)
     finally:
         orc.set_root_mode(orc.ROOTS_EXACT)
-    _cmp(got, ref, "c4 int3c2e_sph")
 
 
 def test_per_call_with_dims(mc, orc, fixture_basis):
