@@ -864,23 +864,26 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
 // summed per block over runs of equal atoms into slot (block % nslot) of
 // the gradient accumulators.
 // ---------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 struct Ip1EpiCfg {
   using C = FastClass<LA, LB, LC, LD, 1>;
+  static constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
+  // LDS budget in doubles: 48 KB, 64 KB for Cartesian output
+  static constexpr int BUDGET = SPH ? 6144 : 8192;
   static constexpr int NEF = C::NE * C::NF;
-  static constexpr int ROWS = 3 * C::NAo * C::NBo;
+  static constexpr int ROWS = 3 * NAo * NBo;
   static constexpr int PER = NEF + ROWS * C::NF;  // doubles of LDS per task
   static constexpr int WORK = C::NF > ROWS ? C::NF : ROWS;
   static constexpr int TE_COVER = WORK > 32 ? 64 : (WORK > 16 ? 32 : (WORK > 8 ? 16 : 8));
-  static constexpr int TPB_LDS = 6144 / PER > 0 ? 6144 / PER : 1;  // <= 48 KB
+  static constexpr int TPB_LDS = BUDGET / PER > 0 ? BUDGET / PER : 1;
   // Lanes per task: the bra pass gives each lane whole columns (three
   // directions each), the digest pass whole rows.  TE_COVER covers the
   // larger of the two; a smaller TE is taken when it fills whole waves,
   // keeps at least 8 waves per CU within the LDS, and halves the estimated
   // wave instructions per task (columns per lane * column cost + rows per
   // lane * row cost, over tasks per wave).
-  static constexpr int COL = 3 * (C::NA * 2 * xfer_terms(LB) + nnz_c2s(LA) * nnz_c2s(LB));
-  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC) * nnz_c2s(LD) + 4 * C::NCo * C::NDo;
+  static constexpr int COL = 3 * (C::NA * 2 * xfer_terms(LB) + nnz_c2s(LA, SPH) * nnz_c2s(LB, SPH));
+  static constexpr int ROW = C::NC * xfer_terms(LD) + nnz_c2s(LC, SPH) * nnz_c2s(LD, SPH) + 4 * NCo * NDo;
   static constexpr int tpb_for(int te) {
     const int t = (256 / te) < TPB_LDS ? 256 / te : TPB_LDS;
     const int q = te < 64 ? 64 / te : 1;  // tasks per full wave
@@ -908,16 +911,16 @@ struct Ip1EpiCfg {
   static constexpr int TE = pick_te();
   static constexpr int TPB = TE == TE_COVER ? ((256 / TE) < TPB_LDS ? 256 / TE : TPB_LDS) : tpb_for(TE);
   static constexpr int THREADS = TPB * TE;
-  static_assert(PER <= 6144, "i-derivative epilogue exceeds its LDS budget");
+  static_assert(PER <= BUDGET, "i-derivative epilogue exceeds its LDS budget");
 };
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool SPH = true>
 __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD, 1>;
-  using P = Ip1EpiCfg<LA, LB, LC, LD>;
+  using P = Ip1EpiCfg<LA, LB, LC, LD, SPH>;
   constexpr int NEP = C::NEP, NF = C::NF, NEF = P::NEF, TE = P::TE, ROWS = P::ROWS;
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
-  constexpr int NAo = C::NAo, NBo = C::NBo, NCo = C::NCo, NDo = C::NDo;
+  constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
   __shared__ double sm[P::TPB * P::PER];
   __shared__ double s_g[P::TPB][3];
   __shared__ int s_atom[P::TPB];
@@ -980,7 +983,7 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
           double s = 0.0;
           sfor<NA * NB>([&](auto AB) {
             constexpr int a = decltype(AB)::value / NB, b = decltype(AB)::value % NB;
-            constexpr double cc = c2sc(LA, ao, a) * c2sc(LB, bo, b);
+            constexpr double cc = c2x(SPH, LA, ao, a) * c2x(SPH, LB, bo, b);
             if constexpr (cc != 0.0) s = fma(cc, Yc[a * NB + b], s);
           });
           Ys[((x * NAo + ao) * NBo + bo) * NF + f] = s;
@@ -996,8 +999,9 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
     const VShell& sc = Bv.vsh[tk.sh[2]];
     const VShell& sd = Bv.vsh[tk.sh[3]];
     int64_t st0, st1, st2, st3;
-    canon_strides(Bv, tk, A.perm, st0, st1, st2, st3);
-    const int64_t stc = (int64_t)sa.dsph * sb.dsph * sc.dsph * sd.dsph;
+    canon_strides<SPH>(Bv, tk, A.perm, st0, st1, st2, st3);
+    const int64_t stc = SPH ? (int64_t)sa.dsph * sb.dsph * sc.dsph * sd.dsph
+                            : (int64_t)sa.dcart * sb.dcart * sc.dcart * sd.dcart;
     double* __restrict__ out = A.out + tk.out;
     if (!any) {
       if (!digest)
@@ -1048,7 +1052,7 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
             double s = 0.0;
             sfor<NC * ND>([&](auto CD) {
               constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
-              constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+              constexpr double cc = c2x(SPH, LC, co, c) * c2x(SPH, LD, dd, d);
               if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
             });
             const int64_t Ri = sc.ao + co, Si = sd.ao + dd;
@@ -1066,7 +1070,7 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
             double s = 0.0;
             sfor<NC * ND>([&](auto CD) {
               constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
-              constexpr double cc = c2sc(LC, co, c) * c2sc(LD, dd, d);
+              constexpr double cc = c2x(SPH, LC, co, c) * c2x(SPH, LD, dd, d);
               if constexpr (cc != 0.0) s = fma(cc, Zc[c * ND + d], s);
             });
             orow[co * st2 + dd * st3] = s;
@@ -1129,10 +1133,14 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
     using P = EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
     hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
-  } else {
+  } else if (A.sph) {
     using P = Ip1EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
     hipLaunchKernelGGL((eri_ip1_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+  } else {
+    using P = Ip1EpiCfg<LA, LB, LC, LD, false>;
+    const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
+    hipLaunchKernelGGL((eri_ip1_epi_kernel<LA, LB, LC, LD, false>), egrid, dim3(P::THREADS), 0, stream, B, A);
   }
   return hipGetLastError();
 }
@@ -1361,7 +1369,7 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
   MCINT_FAST3_CLASSES(MCINT_FAST3_DECLARE)                                                 \
   MCINT_CHUNK3_CLASSES(MCINT_CHUNK3_DECLARE)                                               \
   int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld, double omega) {  \
-    if (!sph && (kind == KIND_2E_IP1 || omega < 0)) return 0;                             \
+    if (!sph && omega < 0) return 0;                                                       \
     if (kind == KIND_2E_IP1) {                                                             \
       MCINT_IP1_CLASSES(MC_CHUNK_HAS)                                                      \
       return 0;                                                                            \

@@ -108,7 +108,7 @@ hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, con
 // Register-resident kernels for the classes listed in eri_fast.h:
 // mode 1 = one kernel with an in-register epilogue, 2 = chunk kernels plus
 // an LDS epilogue (scratch of chunked_block_doubles() per task), 0 = none.
-// Cartesian output: plain 4c and 3c (not the derivative, not short range).
+// Cartesian output: every kind except short range.
 int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld, double omega);
 int chunked_block_doubles(int kind, int la, int lb, int lc, int ld);
 int fast_team_lanes_max();

@@ -146,8 +146,9 @@ def test_synthetic_samples(mc, orc, cfg, n):
     _say(cfg, "sample of", n, "quartets")
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
-        for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph"):
-            qq = q if name == "int2e_sph" else q[: n // 4] if name == "int2e_ip1_sph" else q[: n // 2]
+        for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart"):
+            qq = {"int2e_sph": q, "int2e_cart": q[: n // 2], "int2e_ip1_sph": q[: n // 4],
+                  "int2e_ip1_cart": q[: n // 8]}[name]
             got, _, nz = mc.batch(name, qq, mc.Basis(m.atm, m.bas, m.env))
             ref, _, nz_r = orc.batch(name, qq, orc.Basis(m.atm, m.bas, m.env))
             _cmp(got, ref, f"{cfg} {name}")
