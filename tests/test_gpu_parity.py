@@ -458,3 +458,36 @@ def test_team_shapes_and_screening_walk(mc, orc, ppl):
         mc.tune(mc.TUNE_PRIMS_PER_LANE, 32)
         autotune.apply(autotune.load())
         orc.set_root_mode(orc.ROOTS_EXACT)
+
+
+def test_edge_cases(mc, fixture_basis):
+    """Empty quartet lists and out-of-range shell indices through the C ABI:
+    an empty result, and a clear error before anything is written or
+    launched, for every integral kind; the library stays usable."""
+    import ctypes
+    L = mc.lib()
+    B = mc.Basis(*fixture_basis)
+    P = ctypes.POINTER
+    _say("edge cases: empty lists, bad shell indices")
+
+    def call(name, q):
+        q = np.ascontiguousarray(q, dtype=np.int32).reshape(-1, 4)
+        out = np.full(4096, 7.25)
+        not0 = np.full(max(q.shape[0], 1), -3, np.int32)
+        rc = getattr(L, name + "_batch")(
+            out.ctypes.data_as(P(ctypes.c_double)), None, q.ctypes.data_as(P(ctypes.c_int)),
+            q.shape[0], B.atm.ctypes.data_as(P(ctypes.c_int)), B.natm,
+            B.bas.ctypes.data_as(P(ctypes.c_int)), B.nbas, B.env.ctypes.data_as(P(ctypes.c_double)),
+            None, not0.ctypes.data_as(P(ctypes.c_int)), None)
+        return rc, out, not0
+
+    for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int3c2e_sph"):
+        rc, out, _ = call(name, np.zeros((0, 4)))
+        assert rc == 0, mc.last_error()
+        assert np.all(out == 7.25)
+        for bad in ([0, 0, B.nbas, 0], [0, -1, 0, 0]):
+            rc, out, not0 = call(name, np.array([[0, 0, 0, 0], bad]))
+            assert rc != 0 and "out of range" in mc.last_error(), (name, bad, rc, mc.last_error())
+            assert np.all(out == 7.25) and np.all(not0 == -3)
+    got, _, _ = mc.batch("int2e_sph", np.array([[1, 0, 1, 0]], np.int32), B)
+    assert np.isfinite(got).all() and np.abs(got).max() > 0
