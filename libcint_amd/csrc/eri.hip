@@ -292,17 +292,17 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
 
       const double aij = P.aij;
       const double a1 = aij * akl;
-      const double a0 = a1 / (aij + akl);
+      const double a0 = a1 * rcp_pos(aij + akl);
       const double xpq = P.px - qx, ypq = P.py - qy, zpq = P.pz - qz;
       const double rr = xpq * xpq + ypq * ypq + zpq * zpq;
-      double fac1 = sqrt(a0 / (a1 * a1 * a1)) * cfac * P.kab * kkl;
+      double fac1 = sqrt_pos(a0 * rcp_pos(a1 * a1 * a1)) * cfac * P.kab * kkl;
       double x = a0 * rr;
       double theta = 1.0;
       if (sr) {
         // pairs beyond the unwidened cutoff carry no data (centre at 1e18),
         // and tiny erfc tails are dropped: src/g2e.c:4449-4453
         if (cce_ij >= E || cce_kl >= E) continue;
-        theta = omega * omega / (omega * omega + a0);
+        theta = omega * omega * rcp_pos(omega * omega + a0);
         const double cutoff = (three ? Ecut : Ecut - cce_kl) - cce_ij;
         if (theta * x > cutoff || theta * x > 40.0) continue;  // EXPCUTOFF_SR
         any = 1;
@@ -310,9 +310,9 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         any = 1;
         if (cce_ij >= E || cce_kl >= E) continue;  // stored exp factor is 0
         if (omega > 0) {
-          theta = omega * omega / (omega * omega + a0);
+          theta = omega * omega * rcp_pos(omega * omega + a0);
           x *= theta;
-          fac1 *= sqrt(theta);
+          fac1 *= sqrt_pos(theta);
         }
       }
       if (lane < nrt) {
@@ -321,9 +321,9 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         rys_root_one(nr, lr_part ? lane - nr : lane, lr_part ? x * theta : x, u, w);
         if (omega > 0 || lr_part) {
           const double ut = u * theta;
-          u = ut / (u + 1.0 - ut);
+          u = ut * rcp_pos(u + 1.0 - ut);
         }
-        if (lr_part) w *= -sqrt(theta);
+        if (lr_part) w *= -sqrt_pos(theta);
         s_u[lane] = u;
         s_w[lane] = w * fac1;
       }
@@ -332,7 +332,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         const int d = lane / nrt;
         const int r = lane - d * nrt;
         const double u2 = a0 * s_u[r];
-        const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+        const double tmp4 = 0.5 * rcp_pos(u2 * (aij + akl) + a1);
         const double tmp5 = u2 * tmp4;
         const double tmp1 = 2.0 * tmp5;
         const double b00 = tmp5;

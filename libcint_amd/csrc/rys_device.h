@@ -54,8 +54,8 @@ __device__ __forceinline__ double rys_poly_eval(int n, int f, double x) {
 __device__ __forceinline__ void rys_root_one(int n, int r, double x, double& u, double& w) {
   if (x >= (double)RYS_XMAX(n)) {
     const double rr = rys_lag_r[rys_lag_offset[n] + r];
-    u = rr / (x - rr);
-    w = rys_lag_w[rys_lag_offset[n] + r] / sqrt(x);
+    u = rr * rcp_pos(x - rr);
+    w = rys_lag_w[rys_lag_offset[n] + r] * rsqrt_pos(x);
   } else {
     u = rys_poly_eval(n, r, x);
     w = rys_poly_eval(n, n + r, x);
