@@ -397,7 +397,23 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       sfor<2 * NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
     } else {
       double u[NR], w[NR];
-      rys_roots<NR>(a0 * rr, u, w, rtab);
+      if (Bv.omega > 0) {
+        // long range erf(omega r)/r: the rule at theta x, roots mapped to
+        // u theta / (u + 1 - u theta), weights times sqrt(theta)
+        // (src/g2e.c:4477-4494)
+        const double wl = Bv.omega * Bv.omega;
+        const double th = wl * rcp_pos(wl + a0);
+        rys_roots<NR>(th * (a0 * rr), u, w, rtab);
+        const double sth = sqrt_pos(th);
+        sfor<NR>([&](auto RI) {
+          constexpr int r = decltype(RI)::value;
+          const double ut = u[r] * th;
+          u[r] = ut * rcp_pos(u[r] + 1.0 - ut);
+          w[r] *= sth;
+        });
+      } else {
+        rys_roots<NR>(a0 * rr, u, w, rtab);
+      }
       sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
     }
   }
