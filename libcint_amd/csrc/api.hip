@@ -449,7 +449,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
               (uint8_t)fast_kernel_mode(kind, sph, la, lb, lc, ld, b->omega);
   std::vector<int64_t> first_bad(nthr, INT64_MAX);
   std::vector<int64_t> nprim_thr(nthr, 0), ntask_thr(nthr, 0);
-  const bool reg_ok = b->omega >= 0 || kind == KIND_3C2E;
+  const bool reg_ok = b->omega >= 0 || kind != KIND_2E_IP1;
   auto describe = [&](size_t q, QuartetShape& Q) -> int {
     const FINT* s = shls + 4 * q;
     for (int a = 0; a < 4; ++a) {
@@ -504,8 +504,8 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       const int ld = kind == KIND_3C2E ? 0 : Q.l[L[3]];
       const int64_t nprimq = (int64_t)Q.np[0] * Q.np[1] * Q.np[2] * (naxes == 4 ? Q.np[3] : 1);
       nprim_sum += nprimq;
-      // the short-range four-centre operator runs on the generic kernel;
-      // plain, long-range and short-range three-centre on register kernels
+      // the short-range derivative integrals run on the generic kernel;
+      // everything else has register kernels
       const int mode =
           reg_ok ? mode_tab[((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld] : 0;
       const int ppl_c = g_class_ppl[kind][class_index(la, lb, lc, ld)];

@@ -192,7 +192,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   const double Ax = sa.r[0], Ay = sa.r[1], Az = sa.r[2];
   const double Cx = sc.r[0], Cy = sc.r[1], Cz = sc.r[2];
   constexpr bool THREE = DER == 2 || DER == 3;
-  constexpr bool SR = DER == 3;  // three-centre erfc(omega r)/r, omega < 0
+  constexpr bool SR = DER == 3 || DER == 4;  // erfc(omega r)/r, omega < 0 (3c, 4c)
   const int nij = sa.nprim * sb.nprim;
   const int nkl = THREE ? sc.nprim : sc.nprim * sd.nprim;
   const int64_t pab = (int64_t)tk.sh[0] * Bv.nv + tk.sh[1];
@@ -200,7 +200,7 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   const PairPrim* __restrict__ pij = Bv.pp + Bv.pair_off[pab];
   const PairPrim* __restrict__ pkl = THREE ? nullptr : Bv.pp + Bv.pair_off[pcd];
   const double ld_ij = Bv.pair_ld[pab] * (LA + LB + C::D1);
-  const double ld_kl = THREE ? 0.0 : Bv.pair_ld[pcd] * (LC + LD);
+  double ld_kl = THREE ? 0.0 : Bv.pair_ld[pcd] * (LC + LD);
   const double Ecut = THREE ? Bv.expcutoff_3c : Bv.expcutoff_4c;
   const double* __restrict__ ek = Bv.env + sc.pexp;   // aux exponents (DER = 2)
   const double* __restrict__ ck = Bv.env + sc.pcoef;  // aux coefficients (DER = 2)
@@ -219,18 +219,28 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
   double ldij = ld_ij, Eloop = Ecut;
   [[maybe_unused]] double w2 = 0.0;
   if constexpr (SR) {
-    constexpr int lij = LA + LB, lk = LC;
+    constexpr int lij = LA + LB, lk = LC, lkl = LC + LD;
     w2 = Bv.omega * Bv.omega;
     const double dab = sqrt((Ax - sb.r[0]) * (Ax - sb.r[0]) + (Ay - sb.r[1]) * (Ay - sb.r[1]) +
                             (Az - sb.r[2]) * (Az - sb.r[2]));
     const double th_ij =
         w2 / (w2 + Bv.env[sa.pexp + sa.nprim - 1] + Bv.env[sb.pexp + sb.nprim - 1]);
     ldij = lij > 0 ? lij * log(dab + th_ij * 8.0 + 1.0) : 0.0;
-    if (NR > 1) {
-      if (lij > 0) Eloop += lij * log((dab + th_ij * 8.0 + 1.0) / (dab + 1.0));
-      if (lk > 0) Eloop += lk * log(w2 / (w2 + ek[sc.nprim - 1]) * 8.0 + 1.0);
+    if (NR > 1 && lij > 0) Eloop += lij * log((dab + th_ij * 8.0 + 1.0) / (dab + 1.0));
+    if constexpr (THREE) {
+      if (NR > 1 && lk > 0) Eloop += lk * log(w2 / (w2 + ek[sc.nprim - 1]) * 8.0 + 1.0);
+    } else {
+      // four centres: the ket pair is widened the same way
+      // (src/cint2e.c:358-380)
+      const double dcd = sqrt((Cx - sd.r[0]) * (Cx - sd.r[0]) + (Cy - sd.r[1]) * (Cy - sd.r[1]) +
+                              (Cz - sd.r[2]) * (Cz - sd.r[2]));
+      const double th_kl =
+          w2 / (w2 + Bv.env[sc.pexp + sc.nprim - 1] + Bv.env[sd.pexp + sd.nprim - 1]);
+      ld_kl = lkl > 0 ? lkl * log(dcd + th_kl * 8.0 + 1.0) : 0.0;
+      if (NR > 1 && lkl > 0) Eloop += lkl * log((dcd + th_kl * 8.0 + 1.0) / (dcd + 1.0));
     }
   }
+
 
   // Screening (src/cint2e.c:96-109 and the primitive loops of
   // src/cint2e.c:120-300): a primitive quartet is computed when
@@ -1282,7 +1292,9 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
     const int64_t n = A.t1 - A.t0;                                                         \
     const int64_t per = 256 / A.team;                                                      \
     const dim3 grid((unsigned)((n + per - 1) / per));                                      \
-    if (!A.sph)                                                                            \
+    if (B.omega < 0)                                                                       \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 4>), grid, dim3(256), 0, stream, B, A); \
+    else if (!A.sph)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 0, false>), grid, dim3(256), 0, stream, B, A); \
     else if (A.fp32)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float>), grid, dim3(256), 0, stream, B, A); \
@@ -1294,6 +1306,9 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
   template <>                                                                              \
   hipError_t launch_chunked_class<a, b, c, d>(const BasisView& B, const BucketArgs& A,     \
                                               hipStream_t stream) {                        \
+    if (B.omega < 0)                                                                       \
+      return launch_chunks<a, b, c, d, 4>(                                                 \
+          B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d, 4>::NCH>{});  \
     return launch_chunks<a, b, c, d, 0>(                                                   \
         B, A, stream, std::make_integer_sequence<int, ChunkCfg<a, b, c, d>::NCH>{});       \
   }

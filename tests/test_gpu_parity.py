@@ -493,16 +493,18 @@ def test_edge_cases(mc, fixture_basis):
     assert np.isfinite(got).all() and np.abs(got).max() > 0
 
 
-@pytest.mark.parametrize("name", ["int2e_sph", "int2e_ip1_sph", "int3c2e_sph"])
-def test_long_range_sample_on_register_kernels(mc, orc, name):
-    """erf(0.3 r)/r on a c1 sample (contracted s/p shells, screening) for the
-    plain, derivative and three-centre register kernels against the oracle."""
+@pytest.mark.parametrize("name,omega", [("int2e_sph", 0.3), ("int2e_ip1_sph", 0.3),
+                                        ("int3c2e_sph", 0.3), ("int2e_sph", -0.3)])
+def test_range_separated_sample_on_register_kernels(mc, orc, name, omega):
+    """erf(0.3 r)/r (omega > 0) and erfc(0.3 r)/r (omega < 0) on a c1 sample
+    (contracted s/p shells, screening) for the plain, derivative and
+    three-centre register kernels against the oracle."""
     from libcint_amd.synthetic import make_molecule
     m = make_molecule("c1")
     env = m.env.copy()
-    env[8] = 0.3  # PTR_RANGE_OMEGA
+    env[8] = omega  # PTR_RANGE_OMEGA
     q = _sample_quartets(m.nbas, 900 if name == "int2e_sph" else 300, seed=31)
-    _say("long-range sample", name)
+    _say("range-separated sample", name, omega)
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
         got, off, nz = mc.batch(name, q, mc.Basis(m.atm, m.bas, env))
@@ -511,7 +513,7 @@ def test_long_range_sample_on_register_kernels(mc, orc, name):
         orc.set_root_mode(orc.ROOTS_EXACT)
     if name == "int2e_ip1_sph":
         _cmp_ip1(mc, orc, type("M", (), {"atm": m.atm, "bas": m.bas, "env": env})(), q, got, ref,
-                 off, f"erf {name}")
+                 off, f"omega {omega} {name}")
     else:
-        _cmp(got, ref, f"erf {name}")
+        _cmp(got, ref, f"omega {omega} {name}")
     np.testing.assert_array_equal(nz, nz_r)
