@@ -616,10 +616,14 @@ constexpr int CHUNK_TARGET = 96;
 // Plain classes of at most three roots (la+lb+lc+ld <= 5) split finer: their
 // roots and recurrences are cheap to recompute per chunk, and a block of
 // <= 48 accumulators leaves room for two waves per SIMD (measured on the
-// c1 workload: (pp|pp) 48 -> 38 ms, (dp|pp) 43 -> 33 ms, (fp|p s) 31 -> 25 ms);
-// four-root classes lose more to the recomputation than they gain.
+// c1 workload: (pp|pp) 48 -> 38 ms, (dp|pp) 43 -> 33 ms, (fp|p s) 31 -> 25 ms).
+// Four-root classes with la+lb+lc+ld = 6 gain from <= 64 ((fp|pp) 31 ->
+// 26.5 ms, (dd|pp) 7.1 -> 6.1 ms); at 7 and above the recomputation costs
+// more than the occupancy gains, and 48 lost for all four-root classes.
 constexpr int chunk_target(int la, int lb, int lc, int ld, int der) {
-  return (der == 0 && la + lb + lc + ld <= 5) ? 48 : CHUNK_TARGET;
+  return (der == 0 && la + lb + lc + ld <= 5)   ? 48
+         : (der == 0 && la + lb + lc + ld == 6) ? 64
+                                                : CHUNK_TARGET;
 }
 
 template <int LA, int LB, int LC, int LD, int DER = 0>
