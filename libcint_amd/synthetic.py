@@ -13,7 +13,8 @@ Five workloads are defined, each from a fixed seed:
 
 * ``c0``: C4H8, DZ-like, l <= 2 (48 shells, 96 spherical AOs)
 * ``c1``: C10N3O2H15, TZ-like, l <= 3 (210 shells, 540 spherical AOs)
-* ``c2``: the c0 molecule plus a random symmetric density matrix
+* ``c2``: the c1 molecule (the 30-atom TZ cluster) plus a random symmetric
+  density matrix, for the mixed-precision J/K build
 * ``c3``: C8H12 with the c0 basis plus a d shell on H
 * ``c4``: C16H24, c1-like orbital basis plus an even-tempered auxiliary basis
 """
@@ -237,12 +238,12 @@ def make_molecule(config: str) -> Molecule:
     """Build one of the five synthetic workloads by name ('c0'..'c4')."""
     cid = int(config[1:])
     rng = np.random.default_rng(20260 + cid)
-    if config in ("c0", "c2"):
-        rng = np.random.default_rng(20260)
+    if config == "c0":
         heavy_sym = ["C"] * 4
         nh = 8
         basis = "dz"
-    elif config == "c1":
+    elif config in ("c1", "c2"):
+        rng = np.random.default_rng(20261)
         heavy_sym = ["C"] * 10 + ["N"] * 3 + ["O"] * 2
         nh = 15
         basis = "tz"
@@ -280,7 +281,7 @@ def make_molecule(config: str) -> Molecule:
         aux_start = len(b.bas)
         for ia, s in enumerate(symbols):
             _aux_shells(b, ia, s != "H")
-    name = {"c0": "C4H8/DZ", "c1": "C10N3O2H15/TZ", "c2": "C4H8/DZ+D",
+    name = {"c0": "C4H8/DZ", "c1": "C10N3O2H15/TZ", "c2": "C10N3O2H15/TZ",
             "c3": "C8H12/DZ(d on H)", "c4": "C16H24/TZ+aux"}[config]
     return b.finish(name, symbols, coords, aux_start)
 
@@ -303,6 +304,19 @@ def unique_quartets(nbas: int) -> np.ndarray:
     out[:, 2] = ii[q]
     out[:, 3] = jj[q]
     return out
+
+
+def screened_quartets(q_schwarz: np.ndarray, nshell: int, thresh: float):
+    """Unique quartets (i>=j, k>=l, ij>=kl) of the first nshell shells whose
+    Schwarz bound sqrt((ij|ij)) sqrt((kl|kl)) (q_schwarz[i, j] = sqrt(max
+    |(ij|ij)|)) is at least thresh, in the order of unique_quartets; also
+    returns the number of unique quartets before screening."""
+    ii, jj = np.tril_indices(nshell)
+    qpair = q_schwarz[ii, jj]
+    quart = unique_quartets(nshell)
+    p, r = np.tril_indices(ii.size)
+    keep = qpair[p] * qpair[r] >= thresh
+    return quart[keep], int(quart.shape[0])
 
 
 def subset(mol: Molecule, natoms: int) -> Molecule:

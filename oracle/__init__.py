@@ -36,7 +36,9 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
         for name in ("orc_int2e_sph", "orc_int2e_cart", "orc_int2e_ip1_sph",
-                     "orc_int2e_ip1_cart", "orc_int3c2e_sph", "orc_int3c2e_cart"):
+                     "orc_int2e_ip1_cart", "orc_int3c2e_sph", "orc_int3c2e_cart",
+                     "orc_int3c2e_ip1_sph", "orc_int3c2e_ip1_cart", "orc_int3c2e_ip2_sph",
+                     "orc_int3c2e_ip2_cart", "orc_int2c2e_sph", "orc_int2c2e_cart"):
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp]
@@ -50,6 +52,9 @@ def lib():
         L.orc_sr_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_set_root_mode.argtypes = [ctypes.c_int]
         L.orc_c2s_matrix.argtypes = [ctypes.c_int, dp]
+        L.orc_grad_ip1.restype = ctypes.c_int
+        L.orc_grad_ip1.argtypes = [ip, ctypes.c_int, ip, ctypes.c_int, dp, dp, ctypes.c_int, dp,
+                                   ctypes.c_int]
         L.orc_gto_norm.restype = ctypes.c_double
         L.orc_gto_norm.argtypes = [ctypes.c_int, ctypes.c_double]
         _lib = L
@@ -114,16 +119,25 @@ class Basis:
 
 
 _KIND = {"int2e_sph": 0, "int2e_cart": 1, "int2e_ip1_sph": 2, "int2e_ip1_cart": 3,
-         "int3c2e_sph": 4, "int3c2e_cart": 5}
+         "int3c2e_sph": 4, "int3c2e_cart": 5, "int3c2e_ip1_sph": 6, "int3c2e_ip1_cart": 7,
+         "int3c2e_ip2_sph": 8, "int3c2e_ip2_cart": 9, "int2c2e_sph": 10, "int2c2e_cart": 11}
+
+
+def _shape(name):
+    """(shell slots of shls used, derivative components) of an integral."""
+    ncomp = 3 if "_ip" in name else 1
+    if name.startswith("int2c2e"):
+        return (0, 1), ncomp
+    if name.startswith("int3c2e"):
+        return (0, 1, 2), ncomp
+    return (0, 1, 2, 3), ncomp
 
 
 def intor(name: str, shls, basis: Basis, dims=None):
     """One call of a libcint-named integral; returns (array Fortran-shaped, not0)."""
     sph = not name.endswith("cart")
-    ncomp = 3 if "ip1" in name else 1
-    three = name.startswith("int3c2e")
-    ns = 3 if three else 4
-    d = [basis.shell_dim(int(shls[s]), sph) for s in range(ns)]
+    slots, ncomp = _shape(name)
+    d = [basis.shell_dim(int(shls[s]), sph) for s in slots]
     n = int(np.prod(d)) * ncomp
     out = np.zeros(n)
     sh = np.asarray(list(shls) + [0] * (4 - len(shls)), dtype=np.int32)
@@ -134,15 +148,26 @@ def intor(name: str, shls, basis: Basis, dims=None):
     return out.reshape(shape, order="F"), st
 
 
+def grad_ip1(basis: Basis, dm: np.ndarray, nthreads: int = 0) -> np.ndarray:
+    """dE/dR (natm x 3) of E = 1/2 D J - 1/4 D K from int2e_ip1_sph blocks of
+    every (i, j, k >= l) contracted with the density on the CPU."""
+    dm = np.ascontiguousarray(dm, dtype=np.float64)
+    g = np.zeros((basis.natm, 3))
+    rc = lib().orc_grad_ip1(_ip(basis.atm), basis.natm, _ip(basis.bas), basis.nbas,
+                            _dp(basis.env), _dp(dm), dm.shape[0], _dp(g), int(nthreads))
+    if rc != 0:
+        raise ValueError("orc_grad_ip1: density dimension does not match the basis")
+    return g
+
+
 def batch(name: str, shls: np.ndarray, basis: Basis, nthreads: int = 0):
     """Packed outputs for a list of quartets; returns (out, offsets, not0)."""
     sph = not name.endswith("cart")
-    ncomp = 3 if "ip1" in name else 1
-    three = name.startswith("int3c2e")
+    slots, ncomp = _shape(name)
     shls = np.ascontiguousarray(shls, dtype=np.int32).reshape(-1, 4)
     nq = shls.shape[0]
     dims = np.ones((nq, 4), dtype=np.int64)
-    for s in range(3 if three else 4):
+    for s in slots:
         l = basis.bas[shls[:, s], 1].astype(np.int64)
         nc = basis.bas[shls[:, s], 3].astype(np.int64)
         dims[:, s] = ((2 * l + 1) if sph else (l + 1) * (l + 2) // 2) * nc

@@ -455,7 +455,9 @@ typedef struct {
         double expcutoff;
         double omega;
         int three_center;
-        int deriv_i; /* 1 for int2e_ip1 */
+        int two_center; /* 2c2e: the j slot is an exponent-0 s function too */
+        int deriv_i; /* 1 for int2e_ip1 / int3c2e_ip1 */
+        int deriv_k; /* 1 for int3c2e_ip2 (derivative on the auxiliary shell) */
         int ncomp;
 } OcEnv;
 
@@ -528,34 +530,71 @@ static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas
         oc_strides(e);
 }
 
+/* Three-centre (ij|k), src/g3c2e.c:21-132; deriv_i adds the derivative
+ * on i (int3c2e_ip1, ng = {1,0,0,0,...}, src/autocode/int3c2e.c:80-87),
+ * deriv_k the derivative on the auxiliary shell k (int3c2e_ip2, ng =
+ * {0,0,1,0,...}, :161-168).  two_center: (i|k) of int2c2e
+ * (src/g2c2e.c:15-90, src/cint2c2e.c:351-389), the j slot an exponent-0 s
+ * function without an angular factor. */
 static void oc_init_3c2e(OcEnv *e, const int *shls, const int *atm, const int *bas,
-                         const double *env)
+                         const double *env, int deriv_i, int deriv_k, int two_center)
 {
         memset(e, 0, sizeof(*e));
         e->three_center = 1;
+        e->two_center = two_center;
         for (int s = 0; s < 3; s++) {
+                if (two_center && s == 1) continue;
                 e->l[s] = BAS(ANG_OF, shls[s]);
                 e->nf[s] = (e->l[s] + 1) * (e->l[s] + 2) / 2;
                 e->r[s] = env + ATM(PTR_COORD, BAS(ATOM_OF, shls[s]));
         }
+        if (two_center) {
+                e->l[1] = 0;
+                e->nf[1] = 1;
+                e->r[1] = e->r[0];
+        }
         e->l[3] = 0;
         e->nf[3] = 1;
         e->r[3] = e->r[2];
+        e->deriv_i = deriv_i;
+        e->deriv_k = deriv_k;
         /* the auxiliary shell occupies the 'l' slot of the 4c machinery */
-        e->lc[0] = e->l[0];
+        e->lc[0] = e->l[0] + deriv_i;
         e->lc[1] = e->l[1];
         e->lc[2] = 0;
-        e->lc[3] = e->l[2];
-        e->ncomp = 1;
+        e->lc[3] = e->l[2] + deriv_k;
+        e->ncomp = (deriv_i || deriv_k) ? 3 : 1;
         e->nfall = e->nf[0] * e->nf[1] * e->nf[2];
         e->common_factor = (OC_PI * OC_PI * OC_PI) * 2 / OC_SQRTPI *
-                           common_fac_sp(e->l[0]) * common_fac_sp(e->l[1]) * common_fac_sp(e->l[2]);
+                           common_fac_sp(e->l[0]) * (two_center ? 1.0 : common_fac_sp(e->l[1])) *
+                           common_fac_sp(e->l[2]);
         if (env[PTR_EXPCUTOFF] == 0) e->expcutoff = OC_EXPCUTOFF;
         else e->expcutoff = fmax(OC_MIN_EXPCUTOFF, env[PTR_EXPCUTOFF]);
         e->omega = env[PTR_RANGE_OMEGA];
         e->rys_order = (e->lc[0] + e->lc[1] + e->lc[3]) / 2 + 1;
         e->nroots = e->rys_order * ((e->omega < 0 && e->rys_order <= 3) ? 2 : 1);
         oc_strides(e);
+}
+
+/* nabla on the auxiliary shell of a three-centre g (its 'l' slot):
+ * f(k) = k g(k-1) - 2 ak g(k+1), as src/g2e.c:4550-4591 does for i */
+static void oc_nabla_aux(double *f, const double *g, const OcEnv *e, double ak)
+{
+        int nr = e->nroots;
+        double ak2 = -2 * ak;
+        for (int d = 0; d < 3; d++) {
+                const double *gd = g + d * e->gsize;
+                double *fd = f + d * e->gsize;
+                for (int j = 0; j <= e->l[1]; j++)
+                for (int k = 0; k <= e->l[2]; k++)
+                for (int i = 0; i <= e->l[0]; i++)
+                for (int r = 0; r < nr; r++) {
+                        int p = r + i * e->gsi + k * e->gsl + j * e->gsj;
+                        double v = ak2 * gd[p + e->gsl];
+                        if (k > 0) v += k * gd[p - e->gsl];
+                        fd[p] = v;
+                }
+        }
 }
 
 /* Cartesian exponents in libcint order: lx descending, then ly descending
@@ -897,12 +936,19 @@ static void oc_c2s_4(double *out, const double *gblk, const int *lv, const int *
 /* Driver shared by int2e_{sph,cart}, int2e_ip1_{sph,cart}, int3c2e.   */
 /* src/cint2e.c:57-269 (loop), 794-870 (drv)                           */
 /* ------------------------------------------------------------------ */
-static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int *atm,
-                        const int *bas, const double *env, int deriv_i, int sph, int three_center)
+/* three_center: 1 for (ij|k); 2 for the two-centre (i|k) of int2c2e, whose
+ * shls are (i, k).  deriv_k: derivative on the auxiliary shell (3c only). */
+static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const int *atm,
+                        const int *bas, const double *env, int deriv_i, int sph, int three_center,
+                        int deriv_k)
 {
         OcEnv e;
+        const int two_center = three_center == 2;
+        int shls[4] = {shls_in[0], shls_in[1], shls_in[2], 0};
+        if (two_center) { shls[1] = shls_in[0]; shls[2] = shls_in[1]; }
+        else if (!three_center) shls[3] = shls_in[3];
         int sh[4] = {shls[0], shls[1], shls[2], three_center ? shls[2] : shls[3]};
-        if (three_center) oc_init_3c2e(&e, shls, atm, bas, env);
+        if (three_center) oc_init_3c2e(&e, shls, atm, bas, env, deriv_i, deriv_k, two_center);
         else oc_init_2e(&e, shls, atm, bas, env, deriv_i);
 
         int nprim[4], nctr[4];
@@ -917,6 +963,9 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
         if (three_center) {
                 /* 4th function: s with exponent 0 and unit coefficient */
                 nprim[3] = 1; nctr[3] = 1; ex[3] = &zero_exp; cf[3] = &one_coef;
+        }
+        if (two_center) {
+                nprim[1] = 1; nctr[1] = 1; ex[1] = &zero_exp; cf[1] = &one_coef;
         }
         int ncomp = e.ncomp;
         int nc = nctr[0] * nctr[1] * nctr[2] * nctr[3];
@@ -948,6 +997,15 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
         int empty_ij = oc_pairdata(cce_ij, eij, rij, ex[0], ex[1], cf[0], cf[1], nprim[0], nprim[1],
                                    nctr[0], nctr[1], e.r[0], e.r[1], e.lc[0] + e.lc[1], expcutoff,
                                    e.omega);
+        if (two_center) {
+                /* int2c2e screens no primitive (src/cint2c2e.c:57-147) */
+                empty_ij = 0;
+                for (int ip = 0; ip < nprim[0]; ip++) {
+                        cce_ij[ip] = -1e300;
+                        eij[ip] = 1;
+                        for (int d = 0; d < 3; d++) rij[3 * ip + d] = e.r[0][d];
+                }
+        }
         int empty_kl = 0;
         if (three_center) {
                 /* single centre ket: no pair screening (src/cint3c2e.c:160-211) */
@@ -1007,6 +1065,9 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls, const int
                                 memset(gout, 0, sizeof(double) * nfall * ncomp);
                                 if (deriv_i) {
                                         oc_nabla_i(f, g, &e, ex[0][ip]);
+                                        oc_gout_ip1(gout, g, f, idx, &e);
+                                } else if (deriv_k) {
+                                        oc_nabla_aux(f, g, &e, ex[2][kp]);
                                         oc_gout_ip1(gout, g, f, idx, &e);
                                 } else {
                                         oc_gout_plain(gout, g, idx, &e);
@@ -1075,42 +1136,83 @@ int orc_int2e_sph(double *out, const int *dims, const int *shls, const int *atm,
                   const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 0);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 0, 0);
 }
 int orc_int2e_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
                    const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 0);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 0, 0);
 }
 int orc_int2e_ip1_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
                       const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 1, 0);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 1, 0, 0);
 }
 int orc_int2e_ip1_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
                        const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 0, 0);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 0, 0, 0);
 }
 int orc_int3c2e_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
                     const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 1);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 1, 0);
 }
 int orc_int3c2e_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
                      const int *bas, int nbas, const double *env)
 {
         (void)natm; (void)nbas;
-        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 1);
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 1, 0);
+}
+
+/* int3c2e_ip1 (src/autocode/int3c2e.c:80-95), int3c2e_ip2 (:161-176) and
+ * int2c2e (src/cint2c2e.c:351-389; shls = (i, k)). */
+int orc_int3c2e_ip1_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                        const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 1, 1, 0);
+}
+int orc_int3c2e_ip1_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                         const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 0, 1, 0);
+}
+int orc_int3c2e_ip2_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                        const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 1, 1);
+}
+int orc_int3c2e_ip2_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                         const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 1, 1);
+}
+int orc_int2c2e_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                    const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 2, 0);
+}
+int orc_int2c2e_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                     const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 2, 0);
 }
 
 /* Batch driver shaped like examples/time_c2h6.c:719-745: OpenMP over a list
  * of shell quartets, dynamic schedule.  kind: 0 sph, 1 cart, 2 ip1_sph,
- * 3 ip1_cart, 4 3c2e_sph, 5 3c2e_cart (shls stride 4, 4th ignored for 3c).  out[off[q]] receives
+ * 3 ip1_cart, 4 3c2e_sph, 5 3c2e_cart, 6/7 3c2e_ip1 sph/cart, 8/9 3c2e_ip2
+ * sph/cart, 10/11 2c2e sph/cart (shls stride 4; 4th ignored for 3c, the
+ * first two used for 2c).  out[off[q]] receives
  * quartet q packed.  Returns the number of quartets whose status was < 0. */
 int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_t nq,
               const int *atm, int natm, const int *bas, int nbas, const double *env,
@@ -1130,7 +1232,13 @@ int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_
                 case 2: st = orc_int2e_ip1_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 3: st = orc_int2e_ip1_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 4: st = orc_int3c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
-                default: st = orc_int3c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 5: st = orc_int3c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 6: st = orc_int3c2e_ip1_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 7: st = orc_int3c2e_ip1_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 8: st = orc_int3c2e_ip2_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 9: st = orc_int3c2e_ip2_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 10: st = orc_int2c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                default: st = orc_int2c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 }
                 if (not0) not0[q] = st;
                 if (st < 0) nbad++;
@@ -1145,4 +1253,78 @@ int orc_max_threads(void)
 #else
         return 1;
 #endif
+}
+
+/* Nuclear gradient of E = 1/2 sum D_ij D_kl (ij|kl) - 1/4 sum D_ik D_jl (ij|kl)
+ * restated on the CPU for the tests (libcint leaves this contraction to its
+ * caller; the ip1 blocks are this file's int2e_ip1_sph,
+ * src/autocode/grad2.c:19-88): for every (i, j, k >= l)
+ *   grad[atom(i)][x] += f_kl sum_abcd (d/dr_x a b|c d)
+ *                       [-2 D_ab D_cd + (D_ac D_bd + D_ad D_bc)/2],
+ * f_kl = 2 for k != l, 1 otherwise.  dm: nao x nao spherical density,
+ * row-major.  OpenMP over (i, j), per-thread accumulators. */
+int orc_grad_ip1(const int *atm, int natm, const int *bas, int nbas, const double *env,
+                 const double *dm, int nao, double *grad, int nthreads)
+{
+        int *ao = malloc(sizeof(int) * (nbas + 1));
+        ao[0] = 0;
+        for (int s = 0; s < nbas; s++)
+                ao[s + 1] = ao[s] + (2 * BAS(ANG_OF, s) + 1) * BAS(NCTR_OF, s);
+        if (ao[nbas] != nao) {
+                free(ao);
+                return -1;
+        }
+        for (int i = 0; i < 3 * natm; i++) grad[i] = 0;
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+        {
+                double *g = calloc(3 * natm, sizeof(double));
+                int dmax = 0;
+                for (int s = 0; s < nbas; s++)
+                        if (ao[s + 1] - ao[s] > dmax) dmax = ao[s + 1] - ao[s];
+                double *blk = malloc(sizeof(double) * 3 * dmax * dmax * dmax * dmax);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+                for (int64_t ij = 0; ij < (int64_t)nbas * nbas; ij++) {
+                        const int i = (int)(ij / nbas), j = (int)(ij % nbas);
+                        for (int k = 0; k < nbas; k++)
+                                for (int l = 0; l <= k; l++) {
+                                        const int sh[4] = {i, j, k, l};
+                                        if (!orc_int2e_ip1_sph(blk, NULL, sh, atm, natm, bas, nbas, env))
+                                                continue;
+                                        const int di = ao[i + 1] - ao[i], dj = ao[j + 1] - ao[j];
+                                        const int dk = ao[k + 1] - ao[k], dl = ao[l + 1] - ao[l];
+                                        const int64_t nb = (int64_t)di * dj * dk * dl;
+                                        const double fkl = k != l ? 2.0 : 1.0;
+                                        double s[3] = {0, 0, 0};
+                                        for (int d = 0; d < dl; d++)
+                                        for (int c = 0; c < dk; c++)
+                                        for (int b = 0; b < dj; b++)
+                                        for (int a = 0; a < di; a++) {
+                                                const int P = ao[i] + a, Q = ao[j] + b;
+                                                const int R = ao[k] + c, S = ao[l] + d;
+                                                const double w = -2.0 * dm[(int64_t)P * nao + Q] * dm[(int64_t)R * nao + S]
+                                                        + 0.5 * (dm[(int64_t)P * nao + R] * dm[(int64_t)Q * nao + S]
+                                                                 + dm[(int64_t)P * nao + S] * dm[(int64_t)Q * nao + R]);
+                                                const int64_t e = a + (int64_t)di * (b + (int64_t)dj * (c + (int64_t)dk * d));
+                                                s[0] += blk[e] * w;
+                                                s[1] += blk[nb + e] * w;
+                                                s[2] += blk[2 * nb + e] * w;
+                                        }
+                                        const int at = BAS(ATOM_OF, i);
+                                        for (int x = 0; x < 3; x++) g[3 * at + x] += fkl * s[x];
+                                }
+                }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+                for (int x = 0; x < 3 * natm; x++) grad[x] += g[x];
+                free(blk);
+                free(g);
+        }
+        free(ao);
+        return 0;
 }

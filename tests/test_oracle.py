@@ -8,15 +8,21 @@ sign, index or normalisation, a transposed operand) fails at least one:
   * mathematics: Gauss-rule moment identities against mpmath's incomplete
     gamma, orthonormality of the solid harmonics, the (ss|ss) closed form,
     permutational symmetry, finite differences for the i-derivative, and
-    the (ij|k) = (ij|k s0) identity of the 3-centre integral.
+    the (ij|k) = (ij|k s0) identity of the 3-centre integral;
+  * libcint's Cartesian -> spherical table (src/cart2sph.c:21-806) read as
+    data (tests/golden/libcint_cart2sph.json), entry by entry and with
+    signs, for the oracle and for the device's generated table.
 """
+import json
 import math
+import os
 
 import mpmath as mp
 import numpy as np
 import pytest
 
-from conftest import close_place, libcint_quartets
+from conftest import (ROOT, check_class_fingerprints, class_fingerprints, close_place,
+                      fixture_quartets, libcint_quartets)
 
 
 def fp(a):
@@ -108,6 +114,71 @@ def test_c2s_orthonormal(orc):
     assert abs(d[2, 0] + 0.315391565252520002) < 1e-15
     assert abs(d[4, 3] + 0.546274215296039535) < 1e-15
     np.testing.assert_array_equal(orc.c2s_matrix(1), np.eye(3))
+
+
+def _libcint_c2s():
+    """libcint's g_trans_cart2sph values (src/cart2sph.c:21-806) for l <= 6,
+    committed as data by tests/golden/make_c2s_fixture.py."""
+    with open(os.path.join(ROOT, "tests", "golden", "libcint_cart2sph.json")) as fh:
+        d = json.load(fh)
+    return {int(l): np.array(m) for l, m in d["l"].items()}
+
+
+def _device_c2s_tables():
+    """The device's generated table (libcint_amd/csrc/c2s_tables.inc), parsed."""
+    import re
+    with open(os.path.join(ROOT, "libcint_amd", "csrc", "c2s_tables.inc")) as fh:
+        txt = fh.read()
+    off = [int(x) for x in re.search(r"c2s_offset\[\d+\] = \{([^}]*)\}", txt).group(1).split(",")]
+    coef = np.array([float(x) for x in
+                     re.search(r"c2s_coef\[\d+\] = \{([^}]*)\}", txt).group(1).split(",")
+                     if x.strip()])
+    out = {}
+    for l in range(len(off)):
+        nc = (l + 1) * (l + 2) // 2
+        out[l] = coef[off[l]:off[l] + (2 * l + 1) * nc].reshape(2 * l + 1, nc)
+    return out
+
+
+def _c2s_mismatch(a, b):
+    """Rows of a and b that differ beyond rounding (sign flips included)."""
+    tol = 1e-14 + 1e-13 * np.abs(b)
+    return sorted(set(np.nonzero(np.abs(a - b) > tol)[0].tolist()))
+
+
+def test_c2s_matches_libcint_table(orc):
+    """Oracle and device transform equal libcint's table entry by entry for
+    l = 0..6 (every sign of every row, p as px, py, pz)."""
+    ref = _libcint_c2s()
+    dev = _device_c2s_tables()
+    assert sorted(ref) == list(range(7))
+    for l in range(7):
+        assert _c2s_mismatch(orc.c2s_matrix(l), ref[l]) == [], ("oracle", l)
+        assert _c2s_mismatch(dev[l], ref[l]) == [], ("device table", l)
+
+
+def test_c2s_check_sees_a_flipped_row():
+    """The comparison above fails when one row of an f or g block changes
+    sign (the Condon-Shortley / m<0 convention mistake it guards against)."""
+    ref = _libcint_c2s()
+    for l, row in ((3, 1), (3, 6), (4, 0), (2, 3)):
+        bad = ref[l].copy()
+        bad[row] *= -1
+        assert _c2s_mismatch(bad, ref[l]) == [row]
+
+
+@pytest.mark.parametrize("name", ["int2e_sph", "int2e_cart", "int2e_ip1_sph", "int3c2e_sph"])
+def test_signed_class_fingerprints(orc, fixture_basis, name):
+    """The oracle reproduces the committed signed per-class fingerprints
+    (tests/golden/signed_fingerprints.json; regression pin of the oracle,
+    which the GPU parity suite checks the device against)."""
+    with open(os.path.join(ROOT, "tests", "golden", "signed_fingerprints.json")) as fh:
+        gold = json.load(fh)[name]
+    q = fixture_quartets(name)
+    orc.set_root_mode(orc.ROOTS_EXACT)
+    out, off, _ = orc.batch(name, q, orc.Basis(*fixture_basis))
+    fps, mag = class_fingerprints(fixture_basis[1], q, out, off, name)
+    check_class_fingerprints(gold, fps, mag, name)
 
 
 @pytest.mark.parametrize("mode", ["table", "exact"])
@@ -266,3 +337,94 @@ def test_erf_plus_erfc_is_coulomb(orc, fixture_basis, omega):
         assert np.round(np.abs(d).max(), 9) == 0, (omega, t, np.abs(d).max())
         # the short-range part is a genuine fraction of the full integral
         assert np.abs(res[-omega][t]).max() > 1e-3 * np.abs(res[0.0][t]).max()
+
+
+def _own_atom(m, shell):
+    """atm/bas/env with `shell` moved to a new atom at the same place; returns
+    (atm, bas, env, offset of the new atom's coordinates in env)."""
+    a0 = m.bas[shell, 0]
+    atm = np.vstack([m.atm, m.atm[a0]]).astype(np.int32)
+    env = np.concatenate([m.env, m.env[m.atm[a0, 1]:m.atm[a0, 1] + 3]])
+    atm[-1, 1] = len(m.env)
+    bas = m.bas.copy()
+    bas[shell, 0] = len(atm) - 1
+    return atm, bas, env, len(m.env)
+
+
+@pytest.mark.parametrize("name,slot", [("int3c2e_ip1_cart", 0), ("int3c2e_ip2_cart", 2)])
+def test_3c2e_derivatives_finite_difference(orc, name, slot):
+    """int3c2e_ip1 = <d/dr i> = -d/dA_i and int3c2e_ip2 = <d/dr k> = -d/dC_k of
+    int3c2e_cart, by central differences (h = 1e-4) with the differentiated
+    shell alone on its atom; the c4 cluster's orbital and auxiliary shells."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c4")
+    rng = np.random.default_rng(31 + slot)
+    h = 1e-4
+    done = 0
+    while done < 6:
+        t = [int(rng.integers(0, m.aux_start)), int(rng.integers(0, m.aux_start)),
+             int(rng.integers(m.aux_start, m.nbas))]
+        if t[slot] in [t[s] for s in range(3) if s != slot]:
+            continue
+        done += 1
+        atm, bas, env, pc = _own_atom(m, t[slot])
+        q = (t[0], t[1], t[2], 0)
+        g, _ = orc.intor(name, q, orc.Basis(atm, bas, env))
+        for d in range(3):
+            ep, em = env.copy(), env.copy()
+            ep[pc + d] += h
+            em[pc + d] -= h
+            vp, _ = orc.intor("int3c2e_cart", q, orc.Basis(atm, bas, ep))
+            vm, _ = orc.intor("int3c2e_cart", q, orc.Basis(atm, bas, em))
+            fd = -(vp - vm) / (2 * h)
+            np.testing.assert_allclose(g[..., d], fd, rtol=0,
+                                       atol=2e-7 * max(1.0, np.abs(fd).max()), err_msg=str(q))
+
+
+def test_3c2e_translational_invariance(orc, fixture_basis):
+    """(d/dr i j|k) + (i d/dr j|k) + (i j|d/dr k) = 0 for every triple of the
+    fixture: ties the signs and index orders of int3c2e_ip1 and int3c2e_ip2
+    together (the second term is int3c2e_ip1 of (j, i, k), transposed)."""
+    B = orc.Basis(*fixture_basis)
+    worst = 0.0
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                a, _ = orc.intor("int3c2e_ip1_sph", (i, j, k, 0), B)
+                b, _ = orc.intor("int3c2e_ip1_sph", (j, i, k, 0), B)
+                c, _ = orc.intor("int3c2e_ip2_sph", (i, j, k, 0), B)
+                tot = a + b.transpose(1, 0, 2, 3) + c
+                worst = max(worst, float(np.abs(tot).max()))
+                assert np.abs(tot).max() < 1e-11 * max(1.0, np.abs(a).max()), (i, j, k)
+    assert worst < 1e-10
+
+
+def test_int2c2e_checksum_symmetry_and_3c_identity(orc, golden, fixture_basis):
+    """testsuite/test_3c2e.py:311-319: sum |int2c2e_sph(i, k)| over the four
+    regular shells is 782.3104849606677 (10 places); (i|k) = (k|i)^T; and
+    (i|k) equals int3c2e_sph(i, s0, k) with the fixture's exponent-0 shell."""
+    B = orc.Basis(*fixture_basis)
+    q = np.array([(i, k, 0, 0) for k in range(4) for i in range(4)], np.int32)
+    out, off, nz = orc.batch("int2c2e_sph", q, B)
+    assert np.all(nz == 1)
+    v = float(np.abs(out).sum())
+    assert close_place(v, 782.3104849606677, out.size, 10), v
+    q3 = np.array([(i, 8, k, 0) for k in range(4) for i in range(4)], np.int32)
+    o3, _, _ = orc.batch("int3c2e_sph", q3, B)
+    np.testing.assert_allclose(out, o3, rtol=0, atol=1e-12)
+    for i in range(4):
+        for k in range(4):
+            a, _ = orc.intor("int2c2e_sph", (i, k), B)
+            b, _ = orc.intor("int2c2e_sph", (k, i), B)
+            np.testing.assert_allclose(a, b.T, rtol=0, atol=1e-14 * max(1.0, np.abs(a).max()))
+
+
+@pytest.mark.parametrize("name,ref", [("int3c2e_ip1_sph", 2242.052249221302),
+                                      ("int3c2e_ip2_sph", 1970.982483824248)])
+def test_int3c2e_derivative_checksums(orc, fixture_basis, name, ref):
+    """testsuite/test_3c2e.py:303-307: sum |x| over all (i, j, k) of the four
+    regular shells, 3 components, to 10 places."""
+    q = np.array([(i, j, k, 0) for k in range(4) for j in range(4) for i in range(4)], np.int32)
+    out, _, _ = orc.batch(name, q, orc.Basis(*fixture_basis))
+    v = float(np.abs(out).sum())
+    assert close_place(v, ref, out.size, 10), (name, v, ref)
