@@ -43,39 +43,98 @@ def log(*a):
 
 
 def build_quartets(db, nshell, thresh):
-    from libcint_amd.synthetic import unique_quartets
-    q_s = db.schwarz(nshell)[:nshell, :nshell]
-    ii, jj = np.tril_indices(nshell)
-    qpair = q_s[ii, jj]
-    # pairs ordered as unique_quartets enumerates them
-    quart = unique_quartets(nshell)
-    npair = ii.size
-    p, r = np.tril_indices(npair)
-    keep = qpair[p] * qpair[r] >= thresh
-    return quart[keep], int(quart.shape[0])
+    from libcint_amd.synthetic import screened_quartets
+    return screened_quartets(db.schwarz(nshell), nshell, thresh)
+
+
+def _cpu_info() -> dict:
+    """CPU model, logical CPUs, SMT and the CPUs this process may run on."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        with open("/proc/cpuinfo") as fh:
+            txt = fh.read()
+        for line in txt.splitlines():
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+        sib = [int(x.split(":")[1]) for x in txt.splitlines() if x.startswith("siblings")]
+        cor = [int(x.split(":")[1]) for x in txt.splitlines() if x.startswith("cpu cores")]
+        if sib and cor:
+            info["smt_threads_per_core"] = sib[0] // max(1, cor[0])
+    except OSError:
+        pass
+    return info
 
 
 def cpu_baseline(basis_np, quartets, target_s=12.0, max_threads=16):
-    """Time the CPU oracle (table-mode roots, OpenMP) on a random sample."""
+    """Time the CPU oracle (the restatement of libcint's path, tabulated
+    roots, OpenMP dynamic schedule over quartets) on a random sample of the
+    benchmark's list: with every CPU this process may use (at most 16, one
+    GPU's share of the host) and on one core."""
     import oracle
     oracle.build()
     oracle.set_root_mode(oracle.ROOTS_TABLE)
     B = oracle.Basis(basis_np.atm, basis_np.bas, basis_np.env)
-    nthreads = max(1, min(max_threads, os.cpu_count() or 1))
-    rng = np.random.default_rng(123)
-    n = 256
-    while True:
-        sel = quartets[rng.choice(quartets.shape[0], size=min(n, quartets.shape[0]), replace=False)]
-        t0 = time.perf_counter()
-        out, off, _ = oracle.batch("int2e_sph", sel, B, nthreads=nthreads)
-        dt = time.perf_counter() - t0
-        if dt >= target_s * 0.5 or n >= quartets.shape[0]:
-            break
-        n = int(n * min(8.0, max(2.0, target_s / max(dt, 1e-3))))
-    eris = int(off[-1])
-    return dict(value=eris / dt, unit="ERIs/s", cores=nthreads, kind="port",
-                sample=f"{sel.shape[0]} random quartets of the same list ({eris} ERIs) in {dt:.1f}s, "
-                       f"oracle/cint_oracle.c with tabulated roots, OpenMP dynamic schedule")
+    info = _cpu_info()
+    nthreads = max(1, min(max_threads, info["affinity"]))
+
+    def sample_rate(threads, budget):
+        rng = np.random.default_rng(123)
+        n = 256
+        while True:
+            sel = quartets[rng.choice(quartets.shape[0], size=min(n, quartets.shape[0]),
+                                      replace=False)]
+            t0 = time.perf_counter()
+            _, off, _ = oracle.batch("int2e_sph", sel, B, nthreads=threads)
+            dt = time.perf_counter() - t0
+            if dt >= budget * 0.5 or n >= quartets.shape[0]:
+                return int(off[-1]) / dt, sel.shape[0], int(off[-1]), dt
+            n = int(n * min(8.0, max(2.0, budget / max(dt, 1e-3))))
+
+    rate, nq, eris, dt = sample_rate(nthreads, target_s)
+    rate1, nq1, eris1, dt1 = sample_rate(1, target_s / 2)
+    return dict(value=rate, unit="ERIs/s", cores=nthreads, kind="port",
+                single_core_value=rate1, **info,
+                sample=f"{nq} random quartets of the same list ({eris} ERIs) in {dt:.1f}s on "
+                       f"{nthreads} threads; single core: {nq1} quartets ({eris1} ERIs) in "
+                       f"{dt1:.1f}s; oracle/cint_oracle.c (libcint's algorithm restated in C) "
+                       f"with the GPU's tabulated Rys roots, OpenMP dynamic schedule")
+
+
+def sampled_check(db, B, plan, quart, out, per_bucket=2, seed=7) -> dict:
+    """Blocks of per_bucket tasks at random positions of every bucket, read
+    from the timed run's output, against the same quartets recomputed by the
+    generic kernel (a separate implementation: runtime class, recurrences in
+    LDS; the parity tests pin both to the oracle)."""
+    import torch
+    import libcint_amd as mc
+    rng = np.random.default_rng(seed)
+    picks = []
+    for i, b in enumerate(plan.buckets()):
+        for p0 in rng.choice(int(b["tasks"]), size=min(per_bucket, int(b["tasks"])), replace=False):
+            picks.append(int(plan.bucket_tasks(i, int(p0), 1)[0]))
+    idx = np.unique(np.array(picks, dtype=np.int64))
+    sizes = B.block_sizes("int2e_sph", quart)
+    off = np.zeros(sizes.size + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    pos = np.concatenate([np.arange(off[q], off[q + 1]) for q in idx])
+    got = out[torch.as_tensor(pos, device=out.device)].cpu().numpy()
+    old = mc.tune(mc.TUNE_GENERIC_ONLY, 1)
+    try:
+        gp = mc.Plan(db, "int2e_sph", quart[idx])
+    finally:
+        mc.tune(mc.TUNE_GENERIC_ONLY, old)
+    ref_t = torch.empty(gp.out_size, dtype=torch.float64, device=out.device)
+    gp.execute(ref_t.data_ptr(), graph=False)
+    torch.cuda.synchronize()
+    ref = ref_t.cpu().numpy()
+    gp.close()
+    err = np.abs(got - ref)
+    return {"quartets": int(idx.size), "buckets": len(plan.buckets()),
+            "reference": "generic kernel on the same quartets",
+            "max_abs_err": float(err.max()),
+            "max_rel_err": float((err / np.maximum(np.abs(ref), 1e-300)).max()),
+            "within_1e-12_abs_1e-10_rel": bool(np.all(err <= 1e-12 + 1e-10 * np.abs(ref)))}
 
 
 def gradient_section(reps: int = 3) -> dict:
@@ -166,6 +225,57 @@ def gradient_section(reps: int = 3) -> dict:
     }
 
 
+def c2_section(reps: int = 2) -> dict:
+    """Mixed-precision J and K on the 30-atom C10N3O2H15/TZ cluster (every
+    unique quartet with Schwarz bound >= 1e-12): quartets whose Schwarz
+    bound x max|D| is below 1e-7 run their recurrences and contraction in
+    FP32; roots, weights and the J/K accumulation stay FP64.  Errors are
+    stated against the all-FP64 build on the same list; each build (ERIs +
+    grouped digest) is timed on device-resident data after one untimed run."""
+    import torch
+    import libcint_amd as mc
+    from libcint_amd import jk
+    from libcint_amd.synthetic import make_molecule, random_density, screened_quartets
+
+    m = make_molecule("c2")
+    B = mc.Basis(m.atm, m.bas, m.env)
+    db = mc.DeviceBasis(B)
+    q, _ = screened_quartets(db.schwarz(m.nbas), m.nbas, 1e-12)
+    db.close()
+    dm = random_density(m.nao_sph(), seed=11)
+    dmt = torch.as_tensor(dm, device="cuda")
+    res = {}
+    for tier, thr in (("fp64", 0.0), ("mixed", 1e-7)):
+        jb = jk.JKBuilder(B, q, fp32_threshold=thr, dmax=float(np.abs(dm).max()))
+        st = jb.stats()
+        vj, vk = jb.run(dm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            vj, vk = jb.run(dm)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[tier] = (vj.cpu().numpy(), vk.cpu().numpy(), float(jk.energy(dmt, vj, vk)), dt,
+                     int(st["fp32_tasks"]))
+        jb.close()
+        del vj, vk, jb
+        torch.cuda.empty_cache()
+    j64, k64, e64, t64, _ = res["fp64"]
+    j32, k32, e32, t32, n32 = res["mixed"]
+    return {
+        "workload": f"c2: {m.name}, {m.natm} atoms, {m.nbas} shells, {m.nao_sph()} AOs, "
+                    f"{q.shape[0]} quartets (Schwarz >= 1e-12); D = A A^T / N, seed 11",
+        "jk_fp64_ms": t64 * 1e3,
+        "jk_mixed_ms": t32 * 1e3,
+        "fp32_quartets": n32,
+        "fp32_threshold": 1e-7,
+        "max_abs_dJ": float(np.abs(j32 - j64).max()),
+        "max_abs_dK": float(np.abs(k32 - k64).max()),
+        "energy_fp64": e64,
+        "energy_error": e32 - e64,
+    }
+
+
 def three_centre_section(reps: int = 3) -> dict:
     """int3c2e_sph over every (i >= j | P) triple of the 40-atom C16H24
     cluster (TZ-like orbital basis, aux shells up to g): FP64, the FP32 tier
@@ -237,6 +347,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-gradient", action="store_true",
                     help="skip the J/K, nuclear-gradient and three-centre legs")
+    ap.add_argument("--bucket-table", default="",
+                    help="write every bucket's class, path, tasks, time and FLOP rate (JSON)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -333,6 +445,14 @@ def main():
             traffic = tr["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         traffic = None
+    if args.bucket_table:
+        rows = [dict(bucket=i, cls=list(b["cls"]), perm=b["perm"], team=b["team"], path=b["path"],
+                     tasks=b["tasks"], prim_quartets=int(nprim[i]), ms=btime[i] * 1e3,
+                     model_flops=bflops[i], tflops=bflops[i] / btime[i] / 1e12,
+                     frac=bflops[i] / btime[i] / 1e12 / FP64_PEAK_TFLOPS)
+                for i, b in enumerate(buckets)]
+        with open(args.bucket_table, "w") as fh:
+            json.dump({"config": args.config, "step_ms": ms, "buckets": rows}, fh)
     top = sorted(range(len(buckets)), key=lambda i: -btime[i])[:8]
     for i in top:
         b = buckets[i]
@@ -367,7 +487,9 @@ def main():
         "quartets_per_s": quart.shape[0] * args.steps / dt,
         "model_tflops_per_s": total_flops / (dt / args.steps) / 1e12,
         "roofline": {
-            "bound": "mfma",
+            # vector FP64 FMA (no MFMA instruction is issued): bounded by the
+            # FP64 VALU peak, 78.6 TFLOP/s on MI355X
+            "bound": "fp64-valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -381,12 +503,18 @@ def main():
             "flops_per_launch": dom_flops,
         },
     }
+    log("sampled check of the timed output ...")
+    result["sampled_check"] = sampled_check(db, B, plan, quart, out)
+    log(f"sampled_check: {result['sampled_check']}")
     del out
     plan.close()
     if not args.no_gradient:
         log("gradient / J/K legs ...")
         result["gradient"] = gradient_section()
         log(f"gradient: {result['gradient']}")
+        log("c2 leg (mixed J/K, 30 atoms) ...")
+        result["c2_mixed_jk"] = c2_section()
+        log(f"c2_mixed_jk: {result['c2_mixed_jk']}")
         log("three-centre leg ...")
         result["three_centre"] = three_centre_section()
         log(f"three_centre: {result['three_centre']}")

@@ -68,6 +68,12 @@ typedef struct {
 #define MCINT_INT2E_IP1_CART 3
 #define MCINT_INT3C2E_SPH 4
 #define MCINT_INT3C2E_CART 5
+#define MCINT_INT3C2E_IP1_SPH 6
+#define MCINT_INT3C2E_IP1_CART 7
+#define MCINT_INT3C2E_IP2_SPH 8
+#define MCINT_INT3C2E_IP2_CART 9
+#define MCINT_INT2C2E_SPH 10
+#define MCINT_INT2C2E_CART 11
 
 /* ---- 1. libcint-named per-call entry points --------------------------
  * Signature of libcint's CINTIntegralFunction (include/cint_funcs.h:10-17).
@@ -93,14 +99,48 @@ CACHE_SIZE_T int3c2e_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT na
 CACHE_SIZE_T int3c2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
                           FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
                           /* replaces src/cint3c2e.c:709-717 */
+/* (d/dr i j|k), 3 components, derivative on the first shell: out
+ * (di,dj,dk,3); replaces src/autocode/int3c2e.c:72-87 (cart 72-79, sph 80-87). */
+CACHE_SIZE_T int3c2e_ip1_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                             FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int3c2e_ip1_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                              FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+/* (i j|d/dr k), derivative on the auxiliary shell: out (di,dj,dk,3);
+ * replaces src/autocode/int3c2e.c:153-168 (cart 153-160, sph 161-168). */
+CACHE_SIZE_T int3c2e_ip2_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                             FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int3c2e_ip2_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                              FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+/* Two-centre Coulomb (i|k), shls = {i, k}: out (di,dk); no primitive is
+ * screened.  Replaces src/cint2c2e.c:351-389 (int2c2e_sph, _cart). */
+CACHE_SIZE_T int2c2e_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                         FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int2c2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                          FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
 
 /* Optimizers (src/cint2e.c:1195-1200, src/autocode/grad2.c:69-72,
  * src/cint3c2e.c:703-707) and their destructors (src/optimizer.c:40-72). */
 void int2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void int2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void int3c2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+/* src/autocode/int3c2e.c:68-71, 149-152; src/cint2c2e.c:360-366 */
+void int3c2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void int3c2e_ip2_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void int2c2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void CINTdel_optimizer(CINTOpt **opt);
 void CINTdel_2e_optimizer(CINTOpt **opt);
+/* An optimizer built by the functions above uploads atm/bas/env and the
+ * primitive-pair data of every shell pair to the current device once
+ * (src/optimizer.c:183-191,344-425 builds the CPU counterpart); per-call
+ * functions given that optimizer and the same atm/bas/env then only launch
+ * the tuple's kernel and copy its block back.  Given other data, or an
+ * optimizer without device state, they take the self-contained path
+ * (upload, plan, launch).  Thread safe: concurrent callers sharing one
+ * optimizer each get their own stream and buffers from a pool.
+ * mcint_optimizer_on_device (no libcint counterpart) returns 1 when opt
+ * holds device state, 0 otherwise (no device, or more than 8192 primitives
+ * in the basis). */
+int mcint_optimizer_on_device(const CINTOpt *opt);
 
 /* Legacy names (src/misc.h:34-83, src/c2f.h:6-64) */
 FINT cint2e_sph(double *out, FINT *shls, FINT *atm, FINT natm, FINT *bas, FINT nbas,
@@ -121,9 +161,11 @@ void CINTshells_spheric_offset(FINT ao_loc[], const FINT *bas, const FINT nbas);
 void CINTshells_cart_offset(FINT ao_loc[], const FINT *bas, const FINT nbas);
 
 /* Rys roots u = t^2/(1-t^2) and weights of exp(-x t^2) on [0,1], evaluated
- * on the device (src/rys_roots.c:57-122 computes the same rule).
- * nroots <= 9.  Returns MCINT_OK or an error code. */
-int CINTrys_roots(int nroots, double x, double *u, double *w);
+ * on the device (replaces src/rys_roots.c:57-122, declared at
+ * src/rys_roots.h:3).  nroots <= 9.  On an invalid argument or a device
+ * error u and w are left untouched and mcint_last_error() says why
+ * (libcint calls exit() instead). */
+void CINTrys_roots(int nroots, double x, double *u, double *w);
 int mcint_rys_roots_batch(int nroots, const double *x, size_t n, double *u, double *w);
 
 /* ---- 2. host-pointer batches -------------------------------------------
@@ -151,6 +193,28 @@ int int2e_ip1_cart_batch(double *out, const size_t *out_offsets, const FINT *shl
                          const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
                          const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
 int int3c2e_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                       const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                       const CINTOpt *opt, FINT *not0, void *hip_stream);
+/* Batches of the derivative three-centre and the two-centre integrals
+ * (loops over int3c2e_ip1_*, int3c2e_ip2_*, src/autocode/int3c2e.c:72-87,
+ * 153-168, and int2c2e_*, src/cint2c2e.c:351-374; for int2c2e the first two
+ * entries of each 4-int tuple are (i, k)). */
+int int3c2e_ip1_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                          const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                          const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int3c2e_ip1_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                           const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                           const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int3c2e_ip2_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                          const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                          const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int3c2e_ip2_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                           const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                           const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2c2e_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                      const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                      const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2c2e_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
                        const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
                        const CINTOpt *opt, FINT *not0, void *hip_stream);
 
@@ -199,6 +263,11 @@ int mcint_plan_nbuckets(const mcint_plan *p);
 int mcint_plan_bucket_info(const mcint_plan *p, int bucket, int64_t *info);
 int mcint_plan_execute_bucket(mcint_plan *p, int bucket, double *out_dev, FINT *not0_dev,
                               void *hip_stream);
+/* List positions (indices into the plan's shls) of tasks first..first+n-1
+ * of a bucket, in launch order (host array of n ints; synchronous).  A
+ * generally contracted quartet appears once per contraction column. */
+int mcint_plan_bucket_tasks(const mcint_plan *p, int bucket, int64_t first, int64_t n,
+                            int *quartet);
 /* Primitive quartets that survive libcint's screening, per bucket (host
  * array of nbuckets counts).  Synchronous. */
 int mcint_plan_count_primitives(mcint_plan *p, int64_t *per_bucket);
@@ -247,9 +316,15 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * benchmark's autotuner).  MCINT_TUNE_PRIMS_PER_LANE: the team of lanes
  * that shares one contracted quartet is the smallest power of two (<= 64)
  * giving at most this many primitive quartets per lane (default 32).
+ * MCINT_TUNE_GENERIC_ONLY: value 1 routes every quartet of plans created
+ * afterwards to the generic kernel (runtime class, one workgroup per
+ * quartet, no register kernels): the plain path that the benchmark times
+ * and cross-checks the register kernels against; 0 (default) restores the
+ * class-specialised kernels.
  * Applies to plans created afterwards; returns the previous value, or a
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0
+#define MCINT_TUNE_GENERIC_ONLY 1
 int mcint_tune(int key, int value);
 /* Per-class override of MCINT_TUNE_PRIMS_PER_LANE for one integral kind
  * (intor code) and canonical class (la >= lb, lc >= ld, (la,lb) >= (lc,ld)

@@ -19,15 +19,22 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmcint.so")
 
-INT2E_SPH, INT2E_CART, INT2E_IP1_SPH, INT2E_IP1_CART, INT3C2E_SPH, INT3C2E_CART = range(6)
-INTOR_CODES = {
-    "int2e_sph": INT2E_SPH,
-    "int2e_cart": INT2E_CART,
-    "int2e_ip1_sph": INT2E_IP1_SPH,
-    "int2e_ip1_cart": INT2E_IP1_CART,
-    "int3c2e_sph": INT3C2E_SPH,
-    "int3c2e_cart": INT3C2E_CART,
-}
+INTOR_NAMES = ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
+               "int3c2e_cart", "int3c2e_ip1_sph", "int3c2e_ip1_cart", "int3c2e_ip2_sph",
+               "int3c2e_ip2_cart", "int2c2e_sph", "int2c2e_cart")
+INTOR_CODES = {n: i for i, n in enumerate(INTOR_NAMES)}
+(INT2E_SPH, INT2E_CART, INT2E_IP1_SPH, INT2E_IP1_CART, INT3C2E_SPH, INT3C2E_CART, INT3C2E_IP1_SPH,
+ INT3C2E_IP1_CART, INT3C2E_IP2_SPH, INT3C2E_IP2_CART, INT2C2E_SPH, INT2C2E_CART) = range(12)
+
+
+def intor_shape(name: str):
+    """(slots of a 4-int shell tuple the integral uses, derivative components)."""
+    ncomp = 3 if "_ip" in name else 1
+    if name.startswith("int2c2e"):
+        return (0, 1), ncomp
+    if name.startswith("int3c2e"):
+        return (0, 1, 2), ncomp
+    return (0, 1, 2, 3), ncomp
 
 _lib = None
 
@@ -54,14 +61,12 @@ def lib():
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
     vp = ctypes.c_void_p
-    for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
-                 "int3c2e_cart"):
+    for name in INTOR_NAMES:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, vp, dp]
-    for name in ("int2e_sph_batch", "int2e_cart_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch",
-                 "int2e_ip1_cart_batch", "int3c2e_cart_batch"):
-        f = getattr(L, name)
+    for name in INTOR_NAMES:
+        f = getattr(L, name + "_batch")
         f.restype = ctypes.c_int
         f.argtypes = [dp, ctypes.POINTER(ctypes.c_size_t), ip, ctypes.c_size_t, ip, ctypes.c_int,
                       ip, ctypes.c_int, dp, vp, ip, vp]
@@ -88,6 +93,8 @@ def lib():
     L.mcint_plan_nbuckets.argtypes = [vp]
     L.mcint_plan_bucket_info.restype = ctypes.c_int
     L.mcint_plan_bucket_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.mcint_plan_bucket_tasks.restype = ctypes.c_int
+    L.mcint_plan_bucket_tasks.argtypes = [vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ip]
     L.mcint_plan_execute_bucket.restype = ctypes.c_int
     L.mcint_plan_execute_bucket.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.mcint_plan_count_primitives.restype = ctypes.c_int
@@ -109,7 +116,16 @@ def lib():
     L.mcint_tune_class.restype = ctypes.c_int
     L.mcint_tune_class.argtypes = [ctypes.c_int] * 6
     L.mcint_last_error.restype = ctypes.c_char_p
-    L.CINTrys_roots.restype = ctypes.c_int
+    for name in ("int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer",
+                 "int3c2e_ip1_optimizer", "int3c2e_ip2_optimizer", "int2c2e_optimizer"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_int, ip, ctypes.c_int, dp]
+    L.CINTdel_optimizer.restype = None
+    L.CINTdel_optimizer.argtypes = [ctypes.POINTER(vp)]
+    L.mcint_optimizer_on_device.restype = ctypes.c_int
+    L.mcint_optimizer_on_device.argtypes = [vp]
+    L.CINTrys_roots.restype = None
     L.CINTrys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
     L.mcint_rys_roots_batch.restype = ctypes.c_int
     L.mcint_rys_roots_batch.argtypes = [ctypes.c_int, dp, ctypes.c_size_t, dp, dp]
@@ -124,6 +140,7 @@ def lib():
 
 
 TUNE_PRIMS_PER_LANE = 0
+TUNE_GENERIC_ONLY = 1
 
 
 def tune(key: int, value: int) -> int:
@@ -175,11 +192,10 @@ class Basis:
 
     def block_sizes(self, name: str, shls: np.ndarray) -> np.ndarray:
         sph = not name.endswith("cart")
-        ncomp = 3 if "ip1" in name else 1
-        naxes = 3 if name.startswith("int3c2e") else 4
+        slots, ncomp = intor_shape(name)
         shls = np.asarray(shls).reshape(-1, 4)
         size = np.full(shls.shape[0], ncomp, dtype=np.int64)
-        for a in range(naxes):
+        for a in slots:
             l = self.bas[shls[:, a], 1].astype(np.int64)
             nc = self.bas[shls[:, a], 3].astype(np.int64)
             size *= ((2 * l + 1) if sph else (l + 1) * (l + 2) // 2) * nc
@@ -190,9 +206,8 @@ def intor(name: str, shls, basis: Basis, dims=None):
     """One libcint-style call; returns (Fortran-ordered array, return value)."""
     L = lib()
     sph = not name.endswith("cart")
-    ncomp = 3 if "ip1" in name else 1
-    naxes = 3 if name.startswith("int3c2e") else 4
-    d = [basis.shell_dim(int(shls[a]), sph) for a in range(naxes)]
+    slots, ncomp = intor_shape(name)
+    d = [basis.shell_dim(int(shls[a]), sph) for a in slots]
     out = np.zeros(int(np.prod(d)) * ncomp)
     sh = np.asarray(list(shls) + [0] * (4 - len(shls)), dtype=np.int32)
     rc = getattr(L, name)(_dp(out), None, _ip(sh), _ip(basis.atm), basis.natm, _ip(basis.bas),
@@ -301,6 +316,15 @@ class Plan:
             out.append(dict(cls=tuple(info[:4]), perm=info[4], team=info[5],
                             path=("generic", "fast", "chunked")[info[6]], tasks=info[7]))
         return out
+
+    def bucket_tasks(self, i: int, first: int = 0, n: int | None = None) -> np.ndarray:
+        """List positions of tasks first..first+n-1 of bucket i (launch order)."""
+        if n is None:
+            n = self.buckets()[i]["tasks"] - first
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        _check(lib().mcint_plan_bucket_tasks(self.handle, i, first, n, _ip(out)),
+               "mcint_plan_bucket_tasks")
+        return out[:n]
 
     def count_primitives(self) -> np.ndarray:
         n = lib().mcint_plan_nbuckets(self.handle)

@@ -10,7 +10,9 @@ from __future__ import annotations
 
 import argparse
 import concurrent.futures as cf
+import hashlib
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -40,24 +42,45 @@ if os.environ.get("MCINT_RESOURCE_USAGE"):
     CFLAGS += ["-Rpass-analysis=kernel-resource-usage"]
 
 
-def _headers_mtime() -> float:
-    names = [n for n in os.listdir(CSRC) if n.endswith((".h", ".inc"))]
-    names = [os.path.join(CSRC, n) for n in names]
-    names.append(os.path.join(HERE, "..", "include", "mcint.h"))
-    return max(os.path.getmtime(n) for n in names)
+def _closure(path: str, seen=None) -> list:
+    """The file and every local header it includes (#include "..."), recursively."""
+    seen = [] if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.append(path)
+    with open(path, errors="replace") as fh:
+        for line in fh:
+            m = re.match(r'\s*#\s*include\s+"([^"]+)"', line)
+            if m:
+                _closure(os.path.join(os.path.dirname(path), m.group(1)), seen)
+    return seen
+
+
+def _digest(src: str) -> str:
+    """Hash of the flags and the text of a source and its local includes."""
+    h = hashlib.sha256(" ".join(CFLAGS).encode())
+    for f in _closure(os.path.join(CSRC, src)):
+        with open(f, "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()
 
 
 def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    if incremental and os.path.exists(obj):
-        newest = max(_headers_mtime(), os.path.getmtime(os.path.join(CSRC, src)))
-        if os.path.getmtime(obj) > newest:
-            return obj
+    stamp = obj + ".sha"
+    digest = _digest(src)
+    if incremental and os.path.exists(obj) and os.path.exists(stamp):
+        with open(stamp) as fh:
+            if fh.read().strip() == digest:
+                return obj
     cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     t = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    with open(stamp, "w") as fh:
+        fh.write(digest)
     print(f"  compiled {src} in {time.time() - t:.1f}s", file=sys.stderr)
     if os.environ.get("MCINT_RESOURCE_USAGE"):
         with open(obj + ".remarks.txt", "w") as fh:
@@ -67,7 +90,8 @@ def _compile(src: str, incremental: bool = False) -> str:
 
 def build(jobs: int = 8, incremental: bool = False) -> str:
     """Compile every translation unit and link; incremental=True reuses
-    objects newer than their source and every header."""
+    objects built from the same flags and the same text of the source and
+    of every local header it includes."""
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found")
     os.makedirs(OBJ, exist_ok=True)

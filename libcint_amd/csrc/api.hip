@@ -19,12 +19,14 @@
 #include <cstring>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/mcint.h"
 #include "eri_launch.h"
+#include "host_internal.h"
 
 using namespace mcint;
 
@@ -64,13 +66,18 @@ int intor_kind(int intor) {
     case MCINT_INT2E_IP1_CART: return KIND_2E_IP1;
     case MCINT_INT3C2E_SPH:
     case MCINT_INT3C2E_CART: return KIND_3C2E;
+    case MCINT_INT3C2E_IP1_SPH:
+    case MCINT_INT3C2E_IP1_CART: return KIND_3C2E_IP1;
+    case MCINT_INT3C2E_IP2_SPH:
+    case MCINT_INT3C2E_IP2_CART: return KIND_3C2E_IP2;
+    case MCINT_INT2C2E_SPH:
+    case MCINT_INT2C2E_CART: return KIND_2C2E;
     default: return -1;
   }
 }
 
-int intor_sph(int intor) {
-  return intor == MCINT_INT2E_SPH || intor == MCINT_INT2E_IP1_SPH || intor == MCINT_INT3C2E_SPH;
-}
+// spherical output: the even intor codes
+int intor_sph(int intor) { return intor >= 0 && intor % 2 == 0; }
 
 }  // namespace
 
@@ -83,6 +90,8 @@ struct mcint_basis {
   std::vector<double> env;
   std::vector<VShell> vsh;
   std::vector<int> vfirst;  // first virtual shell of each real shell
+  int dummy = 0;            // virtual shell of the exponent-0 s function (int2c2e's partner)
+  size_t envlen = 0;        // doubles of the caller's env copied (env holds 2 more)
   VShell* vsh_d = nullptr;
   double* env_d = nullptr;
   int* ao_loc_d = nullptr;  // spherical AO offsets of the real shells, nbas+1
@@ -152,6 +161,26 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
       v.ao = ao_loc[s] + c * nsph(l);
       b->vsh.push_back(v);
     }
+  }
+  // the exponent-0 s function that int2c2e pairs with its first shell;
+  // coefficient 2 sqrt(pi) cancels the angular factor of an s function
+  // (libcint's int2c2e carries none for it, src/g2c2e.c:15-90)
+  b->envlen = b->env.size();
+  {
+    VShell v{};
+    v.l = 0;
+    v.nprim = 1;
+    v.pexp = (int)b->envlen;
+    v.pcoef = v.pcoef0 = (int)b->envlen + 1;
+    v.nctr = 1;
+    v.dsph = v.dcart = 1;
+    v.shell = -1;
+    v.atom = -1;
+    v.ao = 0;
+    b->env.push_back(0.0);
+    b->env.push_back(3.5449077018110320546);  // 2 sqrt(pi)
+    b->dummy = (int)b->vsh.size();
+    b->vsh.push_back(v);
   }
   b->nv = (int)b->vsh.size();
   if (b->nv > 32767) {
@@ -241,6 +270,104 @@ struct QuartetShape {
   int64_t so[5];
 };
 
+// Shell data of one shell tuple s (naxes = 3 for int3c2e) and its
+// canonical axis order: 0, or 1 when a shell index is out of range, 2 when
+// the class needs more Rys roots than the device tables hold.
+int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, QuartetShape& Q) {
+  const int naxes = kind_axes(kind);
+  for (int a = 0; a < 4; ++a) {
+    Q.l[a] = 0;
+    Q.w[a] = Q.nc[a] = Q.np[a] = 1;
+  }
+  for (int a = 0; a < naxes; ++a) {
+    if (s[a] < 0 || s[a] >= nbas) return 1;
+    const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
+    Q.l[a] = bs[ANG_OF];
+    Q.nc[a] = bs[NCTR_OF];
+    Q.np[a] = bs[NPRIM_OF];
+    Q.w[a] = sph ? nsph(Q.l[a]) : ncart(Q.l[a]);
+  }
+  const int lsum = Q.l[0] + Q.l[1] + Q.l[2] + Q.l[3] + (kind_ncomp(kind) == 3 ? 1 : 0);
+  if (lsum / 2 + 1 > NRYS_MAX) return 2;
+  Q.so[0] = 1;
+  for (int o = 0; o < 4; ++o) Q.so[o + 1] = Q.so[o] * (int64_t)Q.w[o] * Q.nc[o];
+  // canonical axis order L[A] = original axis
+  for (int a = 0; a < 4; ++a) Q.L[a] = a;
+  int* L = Q.L;
+  const int* l = Q.l;
+  if (kind == KIND_2E) {
+    if (l[L[0]] < l[L[1]]) std::swap(L[0], L[1]);
+    if (l[L[2]] < l[L[3]]) std::swap(L[2], L[3]);
+    if (l[L[2]] > l[L[0]] || (l[L[2]] == l[L[0]] && l[L[3]] > l[L[1]])) {
+      std::swap(L[0], L[2]);
+      std::swap(L[1], L[3]);
+    }
+  } else if (kind == KIND_2E_IP1) {
+    if (l[2] < l[3]) std::swap(L[2], L[3]);
+  } else if (kind == KIND_3C2E || kind == KIND_3C2E_IP2) {
+    if (l[0] < l[1]) std::swap(L[0], L[1]);
+  }
+  // the derivative shell of int3c2e_ip1 stays first; int2c2e keeps (i, k)
+  return 0;
+}
+
+// Canonical class (la, lb, lc, ld) of a described tuple and its axis map
+// (canonical axis A holds the original axis (perm >> 2A) & 3).  int2c2e's
+// canonical axes are (i, exponent-0 s, k): i is original axis 0, k axis 1,
+// and the s function takes the unused axis 2.
+void tuple_class(int kind, const QuartetShape& Q, int& la, int& lb, int& lc, int& ld, int& perm) {
+  const int* L = Q.L;
+  if (kind == KIND_2C2E) {
+    la = Q.l[0];
+    lb = 0;
+    lc = Q.l[1];
+    ld = 0;
+    perm = 0 | (2 << 2) | (1 << 4) | (3 << 6);
+    return;
+  }
+  la = Q.l[L[0]];
+  lb = Q.l[L[1]];
+  lc = Q.l[L[2]];
+  ld = kind_three(kind) ? 0 : Q.l[L[3]];
+  perm = L[0] | (L[1] << 2) | (L[2] << 4) | (L[3] << 6);
+}
+
+// The tasks of one described tuple s: one per combination of contraction
+// columns, in canonical shell order, output offsets from base.
+template <class F>
+void for_each_task(const std::vector<int>& vfirst, int dummy, int kind, const QuartetShape& Q,
+                   const FINT* s, int64_t base, F&& emit) {
+  const int naxes = kind_axes(kind);
+  int cidx[4] = {0, 0, 0, 0};
+  for (;;) {
+    Task t;
+    int64_t o = base;
+    for (int a = 0; a < naxes; ++a) o += (int64_t)cidx[a] * Q.w[a] * Q.so[a];
+    if (kind == KIND_2C2E) {
+      t.sh[0] = (int16_t)(vfirst[s[0]] + cidx[0]);
+      t.sh[1] = (int16_t)dummy;
+      t.sh[2] = t.sh[3] = (int16_t)(vfirst[s[1]] + cidx[1]);
+    } else {
+      for (int A = 0; A < 4; ++A) {
+        if (A < naxes) {
+          const int orig = Q.L[A];
+          t.sh[A] = (int16_t)(vfirst[s[orig]] + cidx[orig]);
+        } else {
+          t.sh[A] = t.sh[naxes - 1];
+        }
+      }
+    }
+    t.out = o;
+    emit(t);
+    int a = 0;
+    for (; a < naxes; ++a) {
+      if (++cidx[a] < Q.nc[a]) break;
+      cidx[a] = 0;
+    }
+    if (a == naxes) break;
+  }
+}
+
 // Set of bucket keys (open addressing; keys are < 2^27, ~0u marks empty).
 struct KeySet {
   std::vector<uint32_t> slot = std::vector<uint32_t>(1024, ~0u);
@@ -310,10 +437,12 @@ constexpr int GRAD_SLOTS = 64;
 // Tuning state (mcint_tune): primitive quartets per lane that the team
 // size aims at.
 int g_prims_per_lane = 32;
+// 1: every quartet on the generic kernel (mcint_tune MCINT_TUNE_GENERIC_ONLY)
+int g_generic_only = 0;
 // per-class overrides (mcint_tune_class), 0 = the global value; indexed by
 // integral kind and canonical class
 constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
-int16_t g_class_ppl[3][NCLS] = {};
+int16_t g_class_ppl[6][NCLS] = {};
 
 inline int class_index(int la, int lb, int lc, int ld) {
   return ((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld;
@@ -338,6 +467,9 @@ struct mcint_plan {
   double* pair_ld_d = nullptr;
   PairPrim* pp_d = nullptr;
   int* pp_pos_d = nullptr;
+  // per-task blocks of the chunked and generic paths and the chunked
+  // path's flags: sizes fixed at creation, allocated on the first execution
+  // that needs them (mcint_plan_execute_noalloc never does)
   double* scratch_d = nullptr;
   int64_t scratch_size = 0;
   int* flags_d = nullptr;
@@ -394,6 +526,15 @@ extern "C" int mcint_tune(int key, int value) {
     g_prims_per_lane = value;
     return old;
   }
+  if (key == MCINT_TUNE_GENERIC_ONLY) {
+    if (value != 0 && value != 1) {
+      set_err("mcint_tune: MCINT_TUNE_GENERIC_ONLY takes 0 or 1");
+      return MCINT_ERR_ARG;
+    }
+    const int old = g_generic_only;
+    g_generic_only = value;
+    return old;
+  }
   set_err("mcint_tune: unknown key");
   return MCINT_ERR_ARG;
 }
@@ -406,14 +547,19 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     return MCINT_ERR_ARG;
   }
   *out = nullptr;
+  if (nq > (size_t)INT32_MAX) {
+    // task ids (the not0 index) are 32-bit on the device
+    set_err("mcint_plan_create: more than 2^31-1 shell tuples in one plan");
+    return MCINT_ERR_ARG;
+  }
   const int kind = intor_kind(intor);
   if (kind < 0) {
     set_err("mcint_plan_create: unknown integral kind");
     return MCINT_ERR_ARG;
   }
   const int sph = intor_sph(intor);
-  const int ncomp = kind == KIND_2E_IP1 ? 3 : 1;
-  const int naxes = kind == KIND_3C2E ? 3 : 4;
+  const int ncomp = kind_ncomp(kind);
+  const int naxes = kind_axes(kind);
   const int* bas = b->bas.data();
   const int nbas = b->nbas;
 
@@ -449,42 +595,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
               (uint8_t)fast_kernel_mode(kind, sph, la, lb, lc, ld, b->omega);
   std::vector<int64_t> first_bad(nthr, INT64_MAX);
   std::vector<int64_t> nprim_thr(nthr, 0), ntask_thr(nthr, 0);
-  const bool reg_ok = b->omega >= 0 || kind != KIND_2E_IP1;
+  const bool reg_ok = (b->omega >= 0 || kind != KIND_2E_IP1) && !g_generic_only;
   auto describe = [&](size_t q, QuartetShape& Q) -> int {
-    const FINT* s = shls + 4 * q;
-    for (int a = 0; a < 4; ++a) {
-      Q.l[a] = 0;
-      Q.w[a] = Q.nc[a] = Q.np[a] = 1;
-    }
-    for (int a = 0; a < naxes; ++a) {
-      if (s[a] < 0 || s[a] >= nbas) return 1;
-      const int* bs = bas + (size_t)s[a] * BAS_SLOTS;
-      Q.l[a] = bs[ANG_OF];
-      Q.nc[a] = bs[NCTR_OF];
-      Q.np[a] = bs[NPRIM_OF];
-      Q.w[a] = sph ? nsph(Q.l[a]) : ncart(Q.l[a]);
-    }
-    const int lsum = Q.l[0] + Q.l[1] + Q.l[2] + Q.l[3] + (kind == KIND_2E_IP1 ? 1 : 0);
-    if (lsum / 2 + 1 > NRYS_MAX) return 2;
-    Q.so[0] = 1;
-    for (int o = 0; o < 4; ++o) Q.so[o + 1] = Q.so[o] * (int64_t)Q.w[o] * Q.nc[o];
-    // canonical axis order L[A] = original axis
-    for (int a = 0; a < 4; ++a) Q.L[a] = a;
-    int* L = Q.L;
-    const int* l = Q.l;
-    if (kind == KIND_2E) {
-      if (l[L[0]] < l[L[1]]) std::swap(L[0], L[1]);
-      if (l[L[2]] < l[L[3]]) std::swap(L[2], L[3]);
-      if (l[L[2]] > l[L[0]] || (l[L[2]] == l[L[0]] && l[L[3]] > l[L[1]])) {
-        std::swap(L[0], L[2]);
-        std::swap(L[1], L[3]);
-      }
-    } else if (kind == KIND_2E_IP1) {
-      if (l[2] < l[3]) std::swap(L[2], L[3]);
-    } else {
-      if (l[0] < l[1]) std::swap(L[0], L[1]);
-    }
-    return 0;
+    return describe_tuple(bas, nbas, kind, sph, shls + 4 * q, Q);
   };
   parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
     // per-thread totals kept in registers (the shared arrays are written
@@ -498,10 +611,8 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
         qnt[q] = 0;
         continue;
       }
-      const int* L = Q.L;
-      const int perm = L[0] | (L[1] << 2) | (L[2] << 4) | (L[3] << 6);
-      const int la = Q.l[L[0]], lb = Q.l[L[1]], lc = Q.l[L[2]];
-      const int ld = kind == KIND_3C2E ? 0 : Q.l[L[3]];
+      int la, lb, lc, ld, perm;
+      tuple_class(kind, Q, la, lb, lc, ld, perm);
       const int64_t nprimq = (int64_t)Q.np[0] * Q.np[1] * Q.np[2] * (naxes == 4 ? Q.np[3] : 1);
       nprim_sum += nprimq;
       // the short-range derivative integrals run on the generic kernel;
@@ -627,30 +738,11 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       const FINT* s = shls + 4 * q;
       int64_t& d = cur[qkey[q]];
       // virtual expansion over contraction columns of the original axes
-      int cidx[4] = {0, 0, 0, 0};
-      for (;;) {
-        Task t;
-        int64_t o = qoff[q];
-        for (int a = 0; a < naxes; ++a) o += (int64_t)cidx[a] * Q.w[a] * Q.so[a];
-        for (int A = 0; A < 4; ++A) {
-          if (A < naxes) {
-            const int orig = Q.L[A];
-            t.sh[A] = (int16_t)(b->vfirst[s[orig]] + cidx[orig]);
-          } else {
-            t.sh[A] = t.sh[naxes - 1];
-          }
-        }
-        t.out = o;
+      for_each_task(b->vfirst, b->dummy, kind, Q, s, qoff[q], [&](const Task& t) {
         sorted[d] = t;
         sorted_q[d] = (int)q;
         ++d;
-        int a = 0;
-        for (; a < naxes; ++a) {
-          if (++cidx[a] < Q.nc[a]) break;
-          cidx[a] = 0;
-        }
-        if (a == naxes) break;
-      }
+      });
     }
   });
   timer.mark("task placement");
@@ -679,8 +771,8 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       p->stats[2] += bk.t1 - bk.t0;
     } else if (bk.mode == 0) {
       const GenLayout G = gen_layout(kind, sph, bk.la, bk.lb, bk.lc, bk.ld);
-      const int nr = (bk.la + bk.lb + bk.lc + (kind == KIND_3C2E ? 0 : bk.ld) +
-                      (kind == KIND_2E_IP1 ? 1 : 0)) / 2 + 1;
+      const int nr = (bk.la + bk.lb + bk.lc + (kind_three(kind) ? 0 : bk.ld) +
+                      (kind_ncomp(kind) == 3 ? 1 : 0)) / 2 + 1;
       const int nrt = b->omega < 0 ? 2 * nr : nr;
       if (G.nE > 220 || G.nF > 165 || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
         plan_free(p);
@@ -712,7 +804,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     for (size_t t = t0; t < t1; ++t) {
       const Task& tk = sorted[t];
       mark((size_t)tk.sh[0] * nv + tk.sh[1]);
-      if (kind != KIND_3C2E) mark((size_t)tk.sh[2] * nv + tk.sh[3]);
+      if (!kind_three(kind)) mark((size_t)tk.sh[2] * nv + tk.sh[3]);
     }
   });
   timer.mark("buckets, pair marks");
@@ -737,8 +829,6 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   if (e == hipSuccess) e = dalloc(&p->pp_pos_d, (size_t)std::max<int64_t>(npp, 1));
   if (e == hipSuccess) e = dalloc(&pp_tmp, (size_t)std::max<int64_t>(npp, 1));
   if (e == hipSuccess) e = dalloc(&pairs_d, pairs.size());
-  if (e == hipSuccess && max_gen > 0) e = dalloc(&p->scratch_d, (size_t)max_gen);
-  if (e == hipSuccess && max_flags > 0) e = dalloc(&p->flags_d, (size_t)max_flags);
   p->scratch_size = max_gen;
   p->flags_size = max_flags;
   if (e == hipSuccess && ntasks > 0)
@@ -829,33 +919,68 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   return A;
 }
 
+// Device memory the chunked and generic paths use for per-task blocks
+// (doubles) and the chunked path's per-task flags; buckets with more tasks
+// than fit run in slices, one after the other on the stream.
+struct ScratchView {
+  double* blocks = nullptr;
+  int64_t nblocks = 0;  // doubles
+  int* flags = nullptr;
+  int64_t nflags = 0;
+};
+
+static ScratchView plan_scratch(const mcint_plan* p) {
+  return ScratchView{p->scratch_d, p->scratch_size, p->flags_d, p->flags_size};
+}
+
+// allocates the plan's scratch on first use
+static int ensure_scratch(mcint_plan* p) {
+  if (p->scratch_size > 0 && !p->scratch_d) HIPCHK(dalloc(&p->scratch_d, (size_t)p->scratch_size));
+  if (p->flags_size > 0 && !p->flags_d) HIPCHK(dalloc(&p->flags_d, (size_t)p->flags_size));
+  return MCINT_OK;
+}
+
+// tasks of one bucket that fit the scratch at once (0: not even one)
+static int64_t slice_tasks(const Bucket& bk, const ScratchView& S) {
+  int64_t n = S.nblocks / std::max<int64_t>(1, bk.sstride);
+  if (bk.mode == 2) n = std::min(n, S.nflags);
+  return n;
+}
+
 static int bucket_launch(mcint_plan* p, const Bucket& bk, double* out, FINT* not0,
-                         hipStream_t stream) {
+                         hipStream_t stream, const ScratchView& S) {
   const BasisView B = plan_view(p);
   BucketArgs A = bucket_args(p, bk, out, not0);
   if (bk.mode == 1) {
     HIPCHK(launch_fast(B, A, stream));
-  } else if (bk.mode == 2) {
-    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
-    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
-      A.t0 = t;
-      A.t1 = std::min(bk.t1, t + chunk);
+    return MCINT_OK;
+  }
+  A.scratch = S.blocks;
+  A.flags = S.flags;
+  const int64_t chunk = slice_tasks(bk, S);
+  if (chunk < 1) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "class (%d%d|%d%d): one task needs %lld doubles of scratch, %lld available",
+             bk.la, bk.lb, bk.lc, bk.ld, (long long)bk.sstride, (long long)S.nblocks);
+    set_err(msg);
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+    A.t0 = t;
+    A.t1 = std::min(bk.t1, t + chunk);
+    if (bk.mode == 2)
       HIPCHK(launch_chunked(B, A, stream));
-    }
-  } else {
-    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
-    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
-      A.t0 = t;
-      A.t1 = std::min(bk.t1, t + chunk);
+    else
       HIPCHK(launch_generic(B, A, stream));
-    }
   }
   return MCINT_OK;
 }
 
-static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream) {
+static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream,
+                       const ScratchView& S) {
   for (const Bucket& bk : p->buckets) {
-    const int st = bucket_launch(p, bk, out, not0, stream);
+    const int st = bucket_launch(p, bk, out, not0, stream, S);
     if (st) return st;
   }
   return MCINT_OK;
@@ -877,6 +1002,7 @@ extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int 
     return MCINT_ERR_ARG;
   }
   hipStream_t stream = (hipStream_t)hip_stream;
+  if (const int st = ensure_scratch(p)) return st;
   const BasisView B = plan_view(p);
   // per-block partial sums go to GRAD_SLOTS copies of the accumulators
   // (block % GRAD_SLOTS), reduced into grad_dev at the end
@@ -894,7 +1020,7 @@ extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int 
     A.nao = nao;
     A.nslot = GRAD_SLOTS;
     A.grad_stride = n;
-    const int64_t chunk = std::max<int64_t>(1, p->scratch_size / std::max<int64_t>(1, bk.sstride));
+    const int64_t chunk = std::max<int64_t>(1, slice_tasks(bk, plan_scratch(p)));
     for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
       A.t0 = t;
       A.t1 = std::min(bk.t1, t + chunk);
@@ -1093,13 +1219,31 @@ extern "C" int mcint_plan_bucket_info(const mcint_plan* p, int i, int64_t* info)
   return MCINT_OK;
 }
 
+extern "C" int mcint_plan_bucket_tasks(const mcint_plan* p, int i, int64_t first, int64_t n,
+                                       int* quartet) {
+  if (!p || i < 0 || i >= (int)p->buckets.size() || first < 0 || n < 0 || (n && !quartet)) {
+    set_err("mcint_plan_bucket_tasks: bad argument");
+    return MCINT_ERR_ARG;
+  }
+  const Bucket& b = p->buckets[i];
+  if (first + n > b.t1 - b.t0) {
+    set_err("mcint_plan_bucket_tasks: range beyond the bucket");
+    return MCINT_ERR_ARG;
+  }
+  if (n) HIPCHK(hipMemcpy(quartet, p->task_id_d + b.t0 + first, (size_t)n * sizeof(int),
+                          hipMemcpyDeviceToHost));
+  return MCINT_OK;
+}
+
 extern "C" int mcint_plan_execute_bucket(mcint_plan* p, int i, double* out_dev, FINT* not0_dev,
                                          void* hip_stream) {
   if (!p || i < 0 || i >= (int)p->buckets.size() || !out_dev) {
     set_err("mcint_plan_execute_bucket: bad argument");
     return MCINT_ERR_ARG;
   }
-  return bucket_launch(p, p->buckets[i], out_dev, not0_dev, (hipStream_t)hip_stream);
+  if (const int st = ensure_scratch(p)) return st;
+  return bucket_launch(p, p->buckets[i], out_dev, not0_dev, (hipStream_t)hip_stream,
+                       plan_scratch(p));
 }
 
 extern "C" int mcint_plan_count_primitives(mcint_plan* p, int64_t* per_bucket) {
@@ -1133,7 +1277,8 @@ extern "C" int mcint_plan_execute_direct(mcint_plan* p, double* out_dev, FINT* n
     set_err("mcint_plan_execute: null argument");
     return MCINT_ERR_ARG;
   }
-  return plan_launch(p, out_dev, not0_dev, (hipStream_t)hip_stream);
+  if (const int st = ensure_scratch(p)) return st;
+  return plan_launch(p, out_dev, not0_dev, (hipStream_t)hip_stream, plan_scratch(p));
 }
 
 extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev, void* hip_stream) {
@@ -1142,14 +1287,15 @@ extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev
     return MCINT_ERR_ARG;
   }
   hipStream_t stream = (hipStream_t)hip_stream;
-  if (!stream) return plan_launch(p, out_dev, not0_dev, stream);
+  if (const int st = ensure_scratch(p)) return st;
+  if (!stream) return plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
   if (!p->gexec || p->g_out != out_dev || p->g_not0 != not0_dev || p->g_stream != stream) {
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->gexec = nullptr;
     p->graph = nullptr;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    int st = plan_launch(p, out_dev, not0_dev, stream);
+    int st = plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (st != MCINT_OK) {
@@ -1166,6 +1312,31 @@ extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev
   HIPCHK(hipGraphLaunch(p->gexec, stream));
   return MCINT_OK;
 }
+
+// Every bucket of p on the stream with caller-supplied scratch (blocks of
+// nblocks doubles, nflags ints of flags); rejects, before any launch, a
+// class whose single task does not fit.
+int mcint::plan_launch_scratch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream,
+                               double* blocks, int64_t nblocks, int* flags, int64_t nflags) {
+  ScratchView S;
+  S.blocks = blocks;
+  S.nblocks = nblocks;
+  S.flags = flags;
+  S.nflags = nflags;
+  for (const Bucket& bk : p->buckets) {
+    if (bk.mode != 1 && slice_tasks(bk, S) < 1) {
+      char msg[200];
+      snprintf(msg, sizeof msg,
+               "class (%d%d|%d%d) needs %lld doubles of scratch per task, %lld available",
+               bk.la, bk.lb, bk.lc, bk.ld, (long long)bk.sstride, (long long)S.nblocks);
+      set_err(msg);
+      return MCINT_ERR_UNSUPPORTED;
+    }
+  }
+  return plan_launch(p, out, not0, stream, S);
+}
+
+void mcint::set_error(const char* msg) { set_err(msg); }
 
 // ---------------------------------------------------------------------------
 // Host-pointer batches
@@ -1264,25 +1435,334 @@ extern "C" int int3c2e_cart_batch(double* out, const size_t* out_offsets, const 
   return host_batch(MCINT_INT3C2E_CART, out, out_offsets, shls, nq, atm, natm, bas, nbas, env,
                     not0, stream);
 }
+#define MC_BATCH(name, code)                                                                    \
+  extern "C" int name(double* out, const size_t* out_offsets, const FINT* shls, size_t nq,      \
+                      const FINT* atm, FINT natm, const FINT* bas, FINT nbas, const double* env, \
+                      const CINTOpt* opt, FINT* not0, void* stream) {                           \
+    (void)opt;                                                                                  \
+    return host_batch(code, out, out_offsets, shls, nq, atm, natm, bas, nbas, env, not0,        \
+                      stream);                                                                  \
+  }
+MC_BATCH(int3c2e_ip1_sph_batch, MCINT_INT3C2E_IP1_SPH)
+MC_BATCH(int3c2e_ip1_cart_batch, MCINT_INT3C2E_IP1_CART)
+MC_BATCH(int3c2e_ip2_sph_batch, MCINT_INT3C2E_IP2_SPH)
+MC_BATCH(int3c2e_ip2_cart_batch, MCINT_INT3C2E_IP2_CART)
+MC_BATCH(int2c2e_sph_batch, MCINT_INT2C2E_SPH)
+MC_BATCH(int2c2e_cart_batch, MCINT_INT2C2E_CART)
+#undef MC_BATCH
 
 // ---------------------------------------------------------------------------
 // libcint-named per-call functions
 // ---------------------------------------------------------------------------
 namespace {
 
+// Device state of an optimizer (the GPU's counterpart of libcint's CINTOpt,
+// src/optimizer.c:183-191,344-425): the basis uploaded once, the primitive
+// pair data of every ordered pair of (virtual) shells, and a pool of call
+// contexts, one per host thread calling at the same time.  Per-call
+// integrals with this optimizer only fill a task, launch its kernel and
+// copy the block back.
+struct CallCtx {
+  hipStream_t stream = nullptr;
+  Task* task_h = nullptr;  // pinned staging of the tasks
+  Task* task_d = nullptr;
+  int* tid_d = nullptr;    // task ids: all zero (one shell tuple per call)
+  int ntask_cap = 0;
+  double* out_d = nullptr;
+  double* out_h = nullptr;  // pinned
+  int64_t out_cap = 0;
+  FINT* not0_d = nullptr;
+  FINT* not0_h = nullptr;  // pinned
+  double* scr_d = nullptr;
+  int64_t scr_cap = 0;
+  int* flags_d = nullptr;
+  int flags_cap = 0;
+};
+
+void ctx_free(CallCtx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->task_h) (void)hipHostFree(c->task_h);
+  if (c->out_h) (void)hipHostFree(c->out_h);
+  if (c->not0_h) (void)hipHostFree(c->not0_h);
+  dfree(c->task_d);
+  dfree(c->tid_d);
+  dfree(c->out_d);
+  dfree(c->not0_d);
+  dfree(c->scr_d);
+  dfree(c->flags_d);
+  delete c;
+}
+
+// grows the context's buffers to nt tasks, blk output doubles, nscr scratch
+// doubles and nflag flags
+hipError_t ctx_reserve(CallCtx* c, int nt, int64_t blk, int64_t nscr, int nflag) {
+  hipError_t e = hipSuccess;
+  if (!c->stream) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess && !c->not0_d) e = dalloc(&c->not0_d, 1);
+  if (e == hipSuccess && !c->not0_h) e = hipHostMalloc((void**)&c->not0_h, sizeof(FINT));
+  if (e == hipSuccess && nt > c->ntask_cap) {
+    if (c->task_h) (void)hipHostFree(c->task_h);
+    c->task_h = nullptr;
+    dfree(c->task_d);
+    dfree(c->tid_d);
+    c->ntask_cap = 0;
+    e = hipHostMalloc((void**)&c->task_h, sizeof(Task) * nt);
+    if (e == hipSuccess) e = dalloc(&c->task_d, nt);
+    if (e == hipSuccess) e = dalloc(&c->tid_d, nt);
+    if (e == hipSuccess) e = hipMemset(c->tid_d, 0, sizeof(int) * nt);
+    if (e == hipSuccess) c->ntask_cap = nt;
+  }
+  if (e == hipSuccess && blk > c->out_cap) {
+    if (c->out_h) (void)hipHostFree(c->out_h);
+    c->out_h = nullptr;
+    dfree(c->out_d);
+    c->out_cap = 0;
+    e = hipHostMalloc((void**)&c->out_h, sizeof(double) * blk);
+    if (e == hipSuccess) e = dalloc(&c->out_d, blk);
+    if (e == hipSuccess) c->out_cap = blk;
+  }
+  if (e == hipSuccess && nscr > c->scr_cap) {
+    dfree(c->scr_d);
+    c->scr_cap = 0;
+    e = dalloc(&c->scr_d, nscr);
+    if (e == hipSuccess) c->scr_cap = nscr;
+  }
+  if (e == hipSuccess && nflag > c->flags_cap) {
+    dfree(c->flags_d);
+    c->flags_cap = 0;
+    e = dalloc(&c->flags_d, nflag);
+    if (e == hipSuccess) c->flags_cap = nflag;
+  }
+  return e;
+}
+
+struct OptDevice {
+  mcint_basis* b = nullptr;
+  int64_t* pair_off_d = nullptr;
+  double* pair_ld_d = nullptr;
+  PairPrim* pp_d = nullptr;
+  int* pp_pos_d = nullptr;
+  std::mutex mu;
+  std::vector<CallCtx*> idle, all;
+};
+
+void optdev_free(OptDevice* d) {
+  if (!d) return;
+  for (CallCtx* c : d->all) ctx_free(c);
+  dfree(d->pair_off_d);
+  dfree(d->pair_ld_d);
+  dfree(d->pp_d);
+  dfree(d->pp_pos_d);
+  mcint_basis_destroy(d->b);
+  delete d;
+}
+
+// Optimizers whose shells hold more primitives than this (summed over
+// contraction columns) keep no device pair table: their pair data would
+// exceed ~4 GB, and calls go through the plan path instead (libcint also
+// skips its pair data above a primitive count, src/optimizer.h:8).
+constexpr int64_t OPT_MAX_TOT_PRIM = 8192;
+
+OptDevice* optdev_create(const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                         const double* env) {
+  auto* d = new OptDevice();
+  if (mcint_basis_create(&d->b, atm, natm, bas, nbas, env) != MCINT_OK) {
+    delete d;
+    return nullptr;
+  }
+  const mcint_basis* b = d->b;
+  const int nv = b->nv;
+  int64_t tot = 0;
+  for (const VShell& v : b->vsh) tot += v.nprim;
+  if (tot > OPT_MAX_TOT_PRIM) {
+    optdev_free(d);
+    return nullptr;
+  }
+  std::vector<int64_t> pair_off((size_t)nv * nv);
+  std::vector<int> pairs;
+  pairs.reserve((size_t)2 * nv * nv);
+  int64_t npp = 0;
+  for (int a = 0; a < nv; ++a)
+    for (int c = 0; c < nv; ++c) {
+      pair_off[(size_t)a * nv + c] = npp;
+      npp += (int64_t)b->vsh[a].nprim * b->vsh[c].nprim;
+      pairs.push_back(a);
+      pairs.push_back(c);
+    }
+  int* pairs_d = nullptr;
+  PairPrim* tmp = nullptr;
+  hipError_t e = dalloc(&d->pair_off_d, pair_off.size());
+  if (e == hipSuccess) e = dalloc(&d->pair_ld_d, pair_off.size());
+  if (e == hipSuccess) e = dalloc(&d->pp_d, (size_t)npp);
+  if (e == hipSuccess) e = dalloc(&d->pp_pos_d, (size_t)npp);
+  if (e == hipSuccess) e = dalloc(&tmp, (size_t)npp);
+  if (e == hipSuccess) e = dalloc(&pairs_d, pairs.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(d->pair_off_d, pair_off.data(), pair_off.size() * sizeof(int64_t),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(pairs_d, pairs.data(), pairs.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_pairdata(b->vsh_d, b->env_d, nv, pairs_d, (int)(pairs.size() / 2), d->pair_off_d,
+                        d->pair_ld_d, tmp, d->pp_d, d->pp_pos_d, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  dfree(pairs_d);
+  dfree(tmp);
+  if (e != hipSuccess) {
+    optdev_free(d);
+    return nullptr;
+  }
+  return d;
+}
+
 struct McOpt {
   CINTOpt pub;
   uint64_t magic;
   int kind;
+  OptDevice* dev;  // null when the device set-up was not possible
 };
 constexpr uint64_t OPT_MAGIC = 0x6d63696e744f7074ull;
 
-CACHE_SIZE_T single_call(int intor, double* out, FINT* dims, FINT* shls, FINT* atm, FINT natm,
-                         FINT* bas, FINT nbas, double* env) {
+// the optimizer's device state, if opt is one of ours and was built from
+// the same atm/bas/env as this call's
+OptDevice* opt_device_for(CINTOpt* opt, const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                          const double* env) {
+  if (!opt) return nullptr;
+  auto* o = (McOpt*)opt;
+  if (o->magic != OPT_MAGIC || !o->dev || !atm || !bas || !env) return nullptr;
+  const mcint_basis* b = o->dev->b;
+  if (natm != b->natm || nbas != b->nbas) return nullptr;
+  if (memcmp(atm, b->atm.data(), sizeof(int) * b->atm.size()) ||
+      memcmp(bas, b->bas.data(), sizeof(int) * b->bas.size()) ||
+      memcmp(env, b->env.data(), sizeof(double) * b->envlen))
+    return nullptr;
+  return o->dev;
+}
+
+// One shell tuple on the optimizer's device state.  Returns libcint's
+// status (any primitive survived) or -1 with mcint_last_error() set; the
+// block goes to blk_out (packed, components outermost).
+int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
+  const mcint_basis* b = d->b;
   const int kind = intor_kind(intor);
   const int sph = intor_sph(intor);
-  const int naxes = kind == KIND_3C2E ? 3 : 4;
-  const int ncomp = kind == KIND_2E_IP1 ? 3 : 1;
+  const int naxes = kind_axes(kind);
+  const int ncomp = kind_ncomp(kind);
+  QuartetShape Q;
+  const int why = describe_tuple(b->bas.data(), b->nbas, kind, sph, shls, Q);
+  if (why) {
+    set_err(why == 1 ? "shell index out of range" : "more Rys roots than the device tables hold");
+    return -1;
+  }
+  int la, lb, lc, ld, perm;
+  tuple_class(kind, Q, la, lb, lc, ld, perm);
+  const int64_t nprimq = (int64_t)Q.np[0] * Q.np[1] * Q.np[2] * (naxes == 4 ? Q.np[3] : 1);
+  const bool reg_ok = (b->omega >= 0 || kind != KIND_2E_IP1) && !g_generic_only;
+  const int mode = reg_ok ? fast_kernel_mode(kind, sph, la, lb, lc, ld, b->omega) : 0;
+  const int ppl_c = g_class_ppl[kind][class_index(la, lb, lc, ld)];
+  const int team = mode ? pick_team(nprimq, ppl_c ? ppl_c : g_prims_per_lane) : 1;
+  int nt = 1;
+  for (int a = 0; a < naxes; ++a) nt *= Q.nc[a];
+  const int64_t blk = Q.so[4] * ncomp;
+  int64_t sstride = 0;
+  if (mode == 2) {
+    sstride = chunked_block_doubles(kind, la, lb, lc, ld);
+  } else if (mode == 0) {
+    const GenLayout G = gen_layout(kind, sph, la, lb, lc, ld);
+    const int nr = (la + lb + lc + ld + (ncomp == 3 ? 1 : 0)) / 2 + 1;
+    const int nrt = b->omega < 0 ? 2 * nr : nr;
+    if (G.nE > 220 || G.nF > 165 || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
+      set_err("angular momentum beyond the generic kernel's tables");
+      return -1;
+    }
+    sstride = G.total;
+  }
+  CallCtx* c = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (!d->idle.empty()) {
+      c = d->idle.back();
+      d->idle.pop_back();
+    } else {
+      c = new CallCtx();
+      d->all.push_back(c);
+    }
+  }
+  auto release = [&] {
+    std::lock_guard<std::mutex> lk(d->mu);
+    d->idle.push_back(c);
+  };
+  hipError_t e = ctx_reserve(c, nt, blk, sstride * nt, mode == 2 ? nt : 0);
+  if (e != hipSuccess) {
+    set_err(std::string("per-call context: ") + hipGetErrorString(e));
+    release();
+    return -1;
+  }
+  // virtual tasks over the contraction columns, as the plan's placement
+  {
+    int t = 0;
+    for_each_task(b->vfirst, b->dummy, kind, Q, shls, 0, [&](const Task& tk) { c->task_h[t++] = tk; });
+  }
+  BasisView B;
+  B.vsh = b->vsh_d;
+  B.env = b->env_d;
+  B.pair_off = d->pair_off_d;
+  B.pair_ld = d->pair_ld_d;
+  B.pp = d->pp_d;
+  B.pp_pos = d->pp_pos_d;
+  B.nv = b->nv;
+  B.expcutoff_4c = b->expc4;
+  B.expcutoff_3c = b->expc3;
+  B.omega = b->omega;
+  BucketArgs A{};
+  A.tasks = c->task_d;
+  A.t0 = 0;
+  A.t1 = nt;
+  A.out = c->out_d;
+  A.not0 = c->not0_d;
+  A.task_id = c->tid_d;
+  A.scratch = c->scr_d;
+  A.flags = c->flags_d;
+  A.scratch_stride = sstride;
+  A.la = la;
+  A.lb = lb;
+  A.lc = lc;
+  A.ld = ld;
+  A.kind = kind;
+  A.sph = sph;
+  A.perm = perm;
+  A.team = team;
+  A.fp32 = 0;
+  A.dm = nullptr;
+  A.grad = nullptr;
+  A.nslot = 1;
+  e = hipMemcpyAsync(c->task_d, c->task_h, sizeof(Task) * nt, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = mode == 1 ? launch_fast(B, A, c->stream)
+                  : (mode == 2 ? launch_chunked(B, A, c->stream) : launch_generic(B, A, c->stream));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->out_h, c->out_d, sizeof(double) * blk, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->not0_h, c->not0_d, sizeof(FINT), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  int st = -1;
+  if (e == hipSuccess) {
+    memcpy(blk_out, c->out_h, sizeof(double) * blk);
+    st = *c->not0_h ? 1 : 0;
+  } else {
+    set_err(std::string("per-call launch: ") + hipGetErrorString(e));
+  }
+  release();
+  return st;
+}
+
+CACHE_SIZE_T single_call(int intor, double* out, FINT* dims, FINT* shls, FINT* atm, FINT natm,
+                         FINT* bas, FINT nbas, double* env, CINTOpt* opt) {
+  const int kind = intor_kind(intor);
+  const int sph = intor_sph(intor);
+  const int naxes = kind_axes(kind);
+  const int ncomp = kind_ncomp(kind);
   if (!shls || !bas) {
     set_err("null shls/bas");
     return 0;
@@ -1302,12 +1782,20 @@ CACHE_SIZE_T single_call(int intor, double* out, FINT* dims, FINT* shls, FINT* a
     return (CACHE_SIZE_T)std::min<int64_t>(blk * ncomp * 2 + 64, 0x7fffffff);
   }
   std::vector<double> tmp((size_t)blk * ncomp);
-  FINT q4[4] = {shls[0], shls[1], shls[2], naxes == 4 ? shls[3] : 0};
+  FINT q4[4] = {shls[0], shls[1], naxes > 2 ? shls[2] : 0, naxes == 4 ? shls[3] : 0};
   FINT st = 0;
-  int rc = host_batch(intor, tmp.data(), nullptr, q4, 1, atm, natm, bas, nbas, env, &st, nullptr);
-  if (rc != MCINT_OK) return 0;
-  const FINT* d = dims ? dims : cnt;
-  const int64_t nout = (int64_t)d[0] * d[1] * d[2] * (naxes == 4 ? d[3] : 1);
+  if (OptDevice* d = opt_device_for(opt, atm, natm, bas, nbas, env)) {
+    const int r = opt_tuple(d, intor, q4, tmp.data());
+    if (r < 0) return 0;
+    st = r;
+  } else {
+    int rc = host_batch(intor, tmp.data(), nullptr, q4, 1, atm, natm, bas, nbas, env, &st, nullptr);
+    if (rc != MCINT_OK) return 0;
+  }
+  // the caller's dims hold naxes entries
+  int d[4] = {1, 1, 1, 1};
+  for (int a = 0; a < naxes; ++a) d[a] = dims ? dims[a] : cnt[a];
+  const int64_t nout = (int64_t)d[0] * d[1] * d[2] * d[3];
   for (int c = 0; c < ncomp; ++c)
     for (int l = 0; l < cnt[3]; ++l)
       for (int k = 0; k < cnt[2]; ++k)
@@ -1318,12 +1806,14 @@ CACHE_SIZE_T single_call(int intor, double* out, FINT* dims, FINT* shls, FINT* a
   return st;
 }
 
-void make_opt(CINTOpt** opt, int kind, FINT nbas) {
+void make_opt(CINTOpt** opt, int kind, FINT* atm, FINT natm, FINT* bas, FINT nbas, double* env) {
   if (!opt) return;
   auto* o = (McOpt*)calloc(1, sizeof(McOpt));
   o->pub.nbas = nbas;
   o->magic = OPT_MAGIC;
   o->kind = kind;
+  o->dev = (atm && bas && env && natm > 0 && nbas > 0) ? optdev_create(atm, natm, bas, nbas, env)
+                                                       : nullptr;
   *opt = &o->pub;
 }
 
@@ -1332,9 +1822,8 @@ void make_opt(CINTOpt** opt, int kind, FINT nbas) {
 #define MC_INTOR(name, code)                                                                   \
   extern "C" CACHE_SIZE_T name(double* out, FINT* dims, FINT* shls, FINT* atm, FINT natm,      \
                                FINT* bas, FINT nbas, double* env, CINTOpt* opt, double* cache) { \
-    (void)opt;                                                                                 \
     (void)cache;                                                                               \
-    return single_call(code, out, dims, shls, atm, natm, bas, nbas, env);                      \
+    return single_call(code, out, dims, shls, atm, natm, bas, nbas, env, opt);                 \
   }
 MC_INTOR(int2e_sph, MCINT_INT2E_SPH)
 MC_INTOR(int2e_cart, MCINT_INT2E_CART)
@@ -1342,23 +1831,53 @@ MC_INTOR(int2e_ip1_sph, MCINT_INT2E_IP1_SPH)
 MC_INTOR(int2e_ip1_cart, MCINT_INT2E_IP1_CART)
 MC_INTOR(int3c2e_sph, MCINT_INT3C2E_SPH)
 MC_INTOR(int3c2e_cart, MCINT_INT3C2E_CART)
+MC_INTOR(int3c2e_ip1_sph, MCINT_INT3C2E_IP1_SPH)
+MC_INTOR(int3c2e_ip1_cart, MCINT_INT3C2E_IP1_CART)
+MC_INTOR(int3c2e_ip2_sph, MCINT_INT3C2E_IP2_SPH)
+MC_INTOR(int3c2e_ip2_cart, MCINT_INT3C2E_IP2_CART)
+MC_INTOR(int2c2e_sph, MCINT_INT2C2E_SPH)
+MC_INTOR(int2c2e_cart, MCINT_INT2C2E_CART)
 #undef MC_INTOR
 
-extern "C" void int2e_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
-  make_opt(opt, KIND_2E, nbas);
+extern "C" void int2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                double* env) {
+  make_opt(opt, KIND_2E, atm, natm, bas, nbas, env);
 }
-extern "C" void int2e_ip1_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
-  make_opt(opt, KIND_2E_IP1, nbas);
+extern "C" void int2e_ip1_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                    double* env) {
+  make_opt(opt, KIND_2E_IP1, atm, natm, bas, nbas, env);
 }
-extern "C" void int3c2e_optimizer(CINTOpt** opt, FINT*, FINT, FINT*, FINT nbas, double*) {
-  make_opt(opt, KIND_3C2E, nbas);
+extern "C" void int3c2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                  double* env) {
+  make_opt(opt, KIND_3C2E, atm, natm, bas, nbas, env);
+}
+extern "C" void int3c2e_ip1_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                      double* env) {
+  make_opt(opt, KIND_3C2E_IP1, atm, natm, bas, nbas, env);
+}
+extern "C" void int3c2e_ip2_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                      double* env) {
+  make_opt(opt, KIND_3C2E_IP2, atm, natm, bas, nbas, env);
+}
+extern "C" void int2c2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                  double* env) {
+  make_opt(opt, KIND_2C2E, atm, natm, bas, nbas, env);
 }
 extern "C" void CINTdel_optimizer(CINTOpt** opt) {
   if (!opt || !*opt) return;
-  free((McOpt*)(*opt));
+  auto* o = (McOpt*)(*opt);
+  if (o->magic == OPT_MAGIC) optdev_free(o->dev);
+  o->magic = 0;
+  free(o);
   *opt = nullptr;
 }
 extern "C" void CINTdel_2e_optimizer(CINTOpt** opt) { CINTdel_optimizer(opt); }
+
+extern "C" int mcint_optimizer_on_device(const CINTOpt* opt) {
+  if (!opt) return 0;
+  const auto* o = (const McOpt*)opt;
+  return o->magic == OPT_MAGIC && o->dev != nullptr;
+}
 
 extern "C" FINT cint2e_sph(double* out, FINT* shls, FINT* atm, FINT natm, FINT* bas, FINT nbas,
                            double* env, CINTOpt* opt) {
@@ -1442,8 +1961,11 @@ extern "C" int mcint_rys_roots_batch(int nroots, const double* x, size_t n, doub
   return MCINT_OK;
 }
 
-extern "C" int CINTrys_roots(int nroots, double x, double* u, double* w) {
-  return mcint_rys_roots_batch(nroots, &x, 1, u, w);
+// libcint's signature (src/rys_roots.h:3) returns nothing; on an invalid
+// nroots or x, or a device error, u and w are left untouched and
+// mcint_last_error() says why.
+extern "C" void CINTrys_roots(int nroots, double x, double* u, double* w) {
+  (void)mcint_rys_roots_batch(nroots, &x, 1, u, w);
 }
 
 // ---------------------------------------------------------------------------

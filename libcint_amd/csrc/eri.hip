@@ -167,8 +167,11 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   const int lane = threadIdx.x;
   const Task tk = A.tasks[t];
   const int kind = A.kind;
-  const bool ip1 = kind == KIND_2E_IP1;
-  const bool three = kind == KIND_3C2E;
+  const bool ip1 = kind_bra_der(kind);
+  const bool ip2 = kind_ket_der(kind);
+  const bool three = kind_three(kind);
+  // (i|k): no screening at all (src/cint2c2e.c:57-147)
+  const bool noscreen = kind == KIND_2C2E;
   const int la = A.la, lb = A.lb, lc = A.lc, ld = three ? 0 : A.ld;
   const GenLayout G = gen_layout(kind, A.sph, la, lb, lc, ld);
   double* X = A.scratch + (t - A.t0) * A.scratch_stride;
@@ -215,8 +218,8 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
   const int lij = la + lb + (ip1 ? 1 : 0);
-  const int lkl = lc + ld;
-  const int nr = (la + lb + lc + ld + (ip1 ? 1 : 0)) / 2 + 1;  // rys_order
+  const int lkl = lc + ld + (ip2 ? 1 : 0);
+  const int nr = (lij + lkl) / 2 + 1;  // rys_order
   const double omega = B.omega;
   const bool sr = omega < 0;
   // distance terms of the screening estimate cce = cce0 - l * log(d + 1);
@@ -247,8 +250,9 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
 
   // pair emptiness (libcint returns early when a pair has no primitive
   // below the cutoff: src/cint2e.c:96-109)
-  int live_ij = 0, live_kl = three ? 1 : 0;
-  for (int p = lane; p < nij; p += GEN_LANES) live_ij |= (pij[p].cce0 - dij_term) < E;
+  int live_ij = noscreen ? 1 : 0, live_kl = three ? 1 : 0;
+  if (!noscreen)
+    for (int p = lane; p < nij; p += GEN_LANES) live_ij |= (pij[p].cce0 - dij_term) < E;
   if (!three)
     for (int p = lane; p < nkl; p += GEN_LANES) live_kl |= (pkl[p].cce0 - dkl_term) < E;
   const int live = __syncthreads_and(__syncthreads_or(live_ij) && __syncthreads_or(live_kl));
@@ -287,7 +291,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         kkl = Q.kab;
         if (cce_kl > Ecut) continue;
       }
-      const double cce_ij = P.cce0 - dij_term;
+      const double cce_ij = noscreen ? -1e300 : P.cce0 - dij_term;
       if (cce_ij > (three ? Ecut : Ecut - cce_kl)) continue;
 
       const double aij = P.aij;
@@ -308,7 +312,7 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         any = 1;
       } else {
         any = 1;
-        if (cce_ij >= E || cce_kl >= E) continue;  // stored exp factor is 0
+        if (!noscreen && (cce_ij >= E || cce_kl >= E)) continue;  // stored exp factor is 0
         if (omega > 0) {
           theta = omega * omega * rcp_pos(omega * omega + a0);
           x *= theta;
@@ -362,6 +366,39 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
       }
       __syncthreads();
       const double wai = 2.0 * P.ai;
+      if (ip2) {
+        // (e | d/dr_comp k) = k_comp [e|k-1_comp] - 2 a_k [e|k+1_comp], per
+        // primitive quartet (the auxiliary exponent a_k is akl here)
+        const int F1 = G.F + 1;
+        const double mak2 = -2.0 * akl;
+        for (int c = lane; c < 3 * ncomp_g; c += GEN_LANES) {
+          const int comp = c / ncomp_g;
+          const int ie = (c - comp * ncomp_g) / G.nF;
+          const int jf = c - comp * ncomp_g - ie * G.nF;
+          const int ec = s_ec[ie], fc = s_fc[jf];
+          const int e3[3] = {ec & 15, (ec >> 4) & 15, (ec >> 8) & 15};
+          const int f3[3] = {fc & 15, (fc >> 4) & 15, (fc >> 8) & 15};
+          double g = 0;
+          for (int r = 0; r < nrt; ++r) {
+            double prod = 1.0;
+            for (int d = 0; d < 3; ++d) {
+              const double* V = s_V + (d * nrt + r) * VS + e3[d] * F1;
+              double v;
+              if (d == comp) {
+                v = mak2 * V[f3[d] + 1];
+                if (f3[d] > 0) v += f3[d] * V[f3[d] - 1];
+              } else {
+                v = V[f3[d]];
+              }
+              prod *= v;
+            }
+            g += prod;
+          }
+          X[G.oX0 + ((int64_t)comp * G.NE0 + ie) * G.nF + jf] += g;
+        }
+        __syncthreads();
+        continue;
+      }
       for (int c = lane; c < ncomp_g; c += GEN_LANES) {
         const int ie = c / G.nF;
         const int jf = c - ie * G.nF;
@@ -422,7 +459,10 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
       while ((mb + 1) * (mb + 2) / 2 <= b) ++mb;
       const int bz = b - mb * (mb + 1) / 2, bx = lb - mb, by = mb - bz;
       double v;
-      if (!ip1) {
+      if (ip2) {
+        v = hrr_bra(X + G.oX0 + (int64_t)comp * G.NE0 * G.nF, G.nF, f, G.p_lo, ax, ay, az, bx, by,
+                    bz, AB);
+      } else if (!ip1) {
         v = hrr_bra(X + G.oX0, G.nF, f, G.p_lo, ax, ay, az, bx, by, bz, AB);
       } else {
         // (d/dr_comp a b| = a_comp (a-1|b) - [2 a_i weighted] (a+1|b)
@@ -596,8 +636,9 @@ __global__ void count_prims_kernel(BasisView B, BucketArgs A, unsigned long long
   const int64_t t = A.t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= A.t1) return;
   const Task tk = A.tasks[t];
-  const bool ip1 = A.kind == KIND_2E_IP1;
-  const bool three = A.kind == KIND_3C2E;
+  const bool ip1 = kind_bra_der(A.kind);
+  const bool three = kind_three(A.kind);
+  const bool noscreen = A.kind == KIND_2C2E;
   const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
   const int nij = s0.nprim * s1.nprim;
   const int nkl = three ? s2.nprim : s2.nprim * B.vsh[tk.sh[3]].nprim;
@@ -606,6 +647,10 @@ __global__ void count_prims_kernel(BasisView B, BucketArgs A, unsigned long long
   const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]] * (A.la + A.lb + (ip1 ? 1 : 0));
   const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]] * (A.lc + A.ld);
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
+  if (noscreen) {
+    atomicAdd(counter, (unsigned long long)nij * nkl);
+    return;
+  }
   int live_ij = 0, live_kl = three;
   for (int p = 0; p < nij; ++p) live_ij |= (pij[p].cce0 - ld_ij) < E;
   if (!three)
@@ -637,8 +682,8 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
   if (n <= 0) return hipSuccess;
   // grid.x is limited to 2^31-1 workgroups; callers split larger buckets
   const GenLayout G = gen_layout(A.kind, A.sph, A.la, A.lb, A.lc, A.ld);
-  const int nr = (A.la + A.lb + A.lc + (A.kind == KIND_3C2E ? 0 : A.ld) +
-                  (A.kind == KIND_2E_IP1 ? 1 : 0)) / 2 + 1;
+  const int nr = (A.la + A.lb + A.lc + (kind_three(A.kind) ? 0 : A.ld) +
+                  (kind_bra_der(A.kind) || kind_ket_der(A.kind) ? 1 : 0)) / 2 + 1;
   const int nrt = B.omega < 0 ? 2 * nr : nr;
   const size_t lds = sizeof(double) * 3 * nrt * (G.E + 1) * (G.F + 1);
   if (lds > sizeof(double) * GEN_VCAP) return hipErrorInvalidValue;

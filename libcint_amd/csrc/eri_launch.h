@@ -32,10 +32,11 @@ __host__ __device__ inline int cart_sum(int lo, int hi) {
 
 __host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int lb, int lc, int ld) {
   GenLayout g;
-  const bool ip1 = kind == KIND_2E_IP1;
-  const bool three = kind == KIND_3C2E;
+  const bool ip1 = kind_bra_der(kind);
+  const bool ip2 = kind_ket_der(kind);
+  const bool three = kind_three(kind);
   if (three) ld = 0;
-  g.ncomp = ip1 ? 3 : 1;
+  g.ncomp = kind_ncomp(kind);
   if (ip1) {
     g.p_lo = la + 1;
     g.p_hi = la + lb + 1;
@@ -52,9 +53,12 @@ __host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int l
     g.e_lo = la;
     g.E = la + lb;
   }
-  g.F = lc + ld;
+  // the auxiliary-shell derivative (int3c2e_ip2) runs the recurrence one
+  // step further on the ket and keeps the columns of shell lc, formed per
+  // primitive quartet as k_x [e|k-1_x] - 2 a_k [e|k+1_x]
+  g.F = lc + ld + (ip2 ? 1 : 0);
   g.nE = cart_sum(g.e_lo, g.E);
-  g.nF = cart_sum(lc, g.F);
+  g.nF = ip2 ? ncart(lc) : cart_sum(lc, g.F);
   g.NE0 = cart_sum(g.p_lo, g.p_hi);
   g.NEM = g.m_hi >= g.m_lo ? cart_sum(g.m_lo, g.m_hi) : 0;
   g.nca = ncart(la);
@@ -66,7 +70,7 @@ __host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int l
   g.nCo = sph ? nsph(lc) : g.ncc;
   g.nDo = sph ? nsph(ld) : g.ncd;
   g.oX0 = 0;
-  g.oXM = g.oX0 + (int64_t)g.NE0 * g.nF;
+  g.oXM = g.oX0 + (int64_t)(ip2 ? 3 : 1) * g.NE0 * g.nF;
   g.oH1 = g.oXM + (int64_t)g.NEM * g.nF;
   g.oH2 = g.oH1 + (int64_t)g.ncomp * g.nca * g.ncb * g.nF;
   g.oH3 = g.oH2 + (int64_t)g.ncomp * g.nAo * g.nBo * g.nF;

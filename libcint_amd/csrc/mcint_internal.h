@@ -17,7 +17,29 @@ constexpr double MIN_EXPCUTOFF = 40.0;  // src/cint_config.h.in:30
 constexpr int LMAX = 6;                 // largest shell l handled on the device
 constexpr int NRYS_MAX = 9;             // largest Rys root count of the tables
 
-enum IntorKind : int { KIND_2E = 0, KIND_2E_IP1 = 1, KIND_3C2E = 2 };
+// Integral kinds.  The three-centre kinds and the two-centre one share the
+// ket of (ij|k): one shell on C and an exponent-0 s function on D; 2c2e
+// also puts an exponent-0 s function (the basis's dummy virtual shell) on B.
+enum IntorKind : int {
+  KIND_2E = 0,
+  KIND_2E_IP1 = 1,    // derivative on shell i (int2e_ip1)
+  KIND_3C2E = 2,
+  KIND_3C2E_IP1 = 3,  // derivative on shell i (int3c2e_ip1)
+  KIND_3C2E_IP2 = 4,  // derivative on the auxiliary shell k (int3c2e_ip2)
+  KIND_2C2E = 5,      // (i|k) (int2c2e)
+};
+__host__ __device__ constexpr bool kind_three(int k) { return k >= KIND_3C2E; }
+__host__ __device__ constexpr bool kind_bra_der(int k) {
+  return k == KIND_2E_IP1 || k == KIND_3C2E_IP1;
+}
+__host__ __device__ constexpr bool kind_ket_der(int k) { return k == KIND_3C2E_IP2; }
+__host__ __device__ constexpr int kind_ncomp(int k) {
+  return kind_bra_der(k) || kind_ket_der(k) ? 3 : 1;
+}
+// shells of the caller's tuple: 4, 3 (ij|k) or 2 (i|k)
+__host__ __device__ constexpr int kind_axes(int k) {
+  return k == KIND_2C2E ? 2 : (kind_three(k) ? 3 : 4);
+}
 
 // One "virtual shell": a (shell, contraction column) pair.  Generally
 // contracted shells expand into nctr virtual shells, each a plain

@@ -22,7 +22,7 @@ def _full_eri_oracle(orc, m, nshell):
     from libcint_amd.synthetic import unique_quartets
     B = orc.Basis(m.atm, m.bas, m.env)
     q = unique_quartets(nshell)
-    out, off, _ = orc.batch("int2e_sph", q, B, nthreads=16)
+    out, off, not0 = orc.batch("int2e_sph", q, B, nthreads=16)
     ao = m.ao_loc_sph(nshell)
     nao = int(ao[-1])
     eri = np.zeros((nao, nao, nao, nao))
@@ -35,7 +35,7 @@ def _full_eri_oracle(orc, m, nshell):
                               (sk, sl, si, sj, (2, 3, 0, 1)), (sl, sk, si, sj, (3, 2, 0, 1)),
                               (sk, sl, sj, si, (2, 3, 1, 0)), (sl, sk, sj, si, (3, 2, 1, 0))):
             eri[a, b, c, e] = blk.transpose(t)
-    return eri
+    return eri, (q, out, off, not0)
 
 
 @pytest.fixture(scope="module")
@@ -44,17 +44,17 @@ def c0_setup(orc):
     m = make_molecule("c0")
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
-        eri = _full_eri_oracle(orc, m, m.nbas)
+        eri, raw = _full_eri_oracle(orc, m, m.nbas)
     finally:
         orc.set_root_mode(orc.ROOTS_EXACT)
     dm = random_density(m.nao_sph(), seed=7)
-    return m, eri, dm
+    return m, eri, dm, raw
 
 
 def test_jk_and_energy_vs_oracle(c0_setup):
     import libcint_amd as mc
     from libcint_amd import jk
-    m, eri, dm = c0_setup
+    m, eri, dm, _ = c0_setup
     _say("c0 J/K")
     vj, vk = jk.get_jk(mc.Basis(m.atm, m.bas, m.env), dm)
     vj, vk = vj.cpu().numpy(), vk.cpu().numpy()
@@ -147,22 +147,48 @@ def test_autograd_function_matches_digest():
     np.testing.assert_allclose(dmt.grad.cpu().numpy(), (vj - 0.5 * vk).cpu().numpy(), atol=1e-12)
 
 
+def test_c0_every_unique_quartet_vs_oracle(c0_setup):
+    """All 692,076 unique quartets of the 12-atom cluster: GPU blocks and
+    non-zero flags element by element against the oracle's."""
+    import libcint_amd as mc
+    m, _, _, (q, ref, off, not0) = c0_setup
+    _say("c0 all unique quartets", len(q))
+    got, off_g, nz = mc.batch("int2e_sph", q, mc.Basis(m.atm, m.bas, m.env))
+    np.testing.assert_array_equal(off_g, off)
+    np.testing.assert_array_equal(nz, not0)
+    err = np.abs(got - ref)
+    bad = err > 1e-12 + 1e-10 * np.abs(ref)
+    _say("c0 max |err|", float(err.max()), "elements", got.size)
+    assert not bad.any(), (int(bad.sum()), float(err.max()))
+
+
 def test_mixed_precision_jk(c0_setup):
-    """FP32 tier for Schwarz*max|D| < 1e-7: J, K and energy stay within the
-    stated errors of the all-FP64 result."""
+    """FP32 tier for Schwarz*max|D| < 1e-7: J, K and energy against the
+    oracle's dense contraction (errors stated) and against the all-FP64
+    GPU result."""
     import libcint_amd as mc
     from libcint_amd import jk
-    m, eri, dm = c0_setup
+    m, eri, dm, _ = c0_setup
     B = mc.Basis(m.atm, m.bas, m.env)
     vj, vk = jk.get_jk(B, dm)
     st = {}
     vj32, vk32 = jk.get_jk(B, dm, fp32_threshold=1e-7, stats=st)
     _say("mixed J/K", st)
     assert st["fp32_tasks"] > 0
+    J = np.einsum("ijkl,kl->ij", eri, dm)
+    K = np.einsum("ijkl,jl->ik", eri, dm)
+    e_ref = 0.5 * np.sum(dm * J) - 0.25 * np.sum(dm * K)
+    j32, k32 = vj32.cpu().numpy(), vk32.cpu().numpy()
+    e32 = 0.5 * np.sum(dm * j32) - 0.25 * np.sum(dm * k32)
+    dj_o = float(np.abs(j32 - J).max())
+    dk_o = float(np.abs(k32 - K).max())
+    _say(f"mixed vs oracle: max|dJ| {dj_o:.2e} max|dK| {dk_o:.2e} dE {e32 - e_ref:.2e}")
+    assert dj_o < 1e-8 and dk_o < 1e-8
+    assert abs(e32 - e_ref) < 1e-9
     dj = float((vj32 - vj).abs().max())
     dk = float((vk32 - vk).abs().max())
     de = float(jk.energy(vj.new_tensor(dm), vj32, vk32) - jk.energy(vj.new_tensor(dm), vj, vk))
-    _say(f"max|dJ| {dj:.2e} max|dK| {dk:.2e} dE {de:.2e}")
+    _say(f"mixed vs FP64: max|dJ| {dj:.2e} max|dK| {dk:.2e} dE {de:.2e}")
     assert dj < 1e-8 and dk < 1e-8
     assert abs(de) < 1e-9
 

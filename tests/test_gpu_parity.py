@@ -6,11 +6,13 @@ relative of the exact quad-precision rule.
 """
 import os
 import sys
+import zlib
 
 import numpy as np
 import pytest
 
-from conftest import close_place, libcint_quartets
+from conftest import (ROOT, check_class_fingerprints, class_fingerprints, close_place,
+                      fixture_quartets, libcint_quartets)
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +76,23 @@ def test_libcint_fixture(mc, orc, golden, fixture_basis, name, key):
         assert close_place(v, gref, got.size, place), (v, gref)
 
 
+@pytest.mark.parametrize("name", ["int2e_sph", "int2e_cart", "int2e_ip1_sph", "int3c2e_sph"])
+def test_signed_class_fingerprints(mc, fixture_basis, name):
+    """Device blocks over libcint's fixture against the committed signed
+    per-class fingerprints sum_k x_k cos(k) (computed by the oracle, whose
+    spherical transform is pinned to libcint's table): a sign flip in any
+    spherical row, Cartesian component or derivative direction of any class
+    changes its class's fingerprint."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "signed_fingerprints.json")) as fh:
+        gold = json.load(fh)[name]
+    _say("signed fingerprints", name)
+    q = fixture_quartets(name)
+    out, off, _ = mc.batch(name, q, mc.Basis(*fixture_basis))
+    fps, mag = class_fingerprints(fixture_basis[1], q, out, off, name)
+    check_class_fingerprints(gold, fps, mag, name)
+
+
 def test_libcint_fixture_3c2e(mc, orc, golden, fixture_basis):
     _say("fixture int3c2e_sph")
     q = np.array([(i, j, k, 0) for k in range(4) for j in range(4) for i in range(4)], np.int32)
@@ -92,6 +111,68 @@ def test_libcint_fixture_3c2e_cart(mc, orc, fixture_basis):
     ref, _, nz_r = orc.batch("int3c2e_cart", q, orc.Basis(*fixture_basis))
     _cmp(got, ref, "int3c2e_cart")
     np.testing.assert_array_equal(nz, nz_r)
+
+
+@pytest.mark.parametrize("name,ref", [("int3c2e_ip1_sph", 2242.052249221302),
+                                      ("int3c2e_ip2_sph", 1970.982483824248),
+                                      ("int2c2e_sph", 782.3104849606677)])
+def test_fixture_3c2e_derivatives_and_2c2e(mc, orc, golden, fixture_basis, name, ref):
+    """int3c2e_ip1 / int3c2e_ip2 / int2c2e over testsuite/test_3c2e.py's loops
+    (i, j, k over the four regular shells; (i, k) for int2c2e): element by
+    element against the oracle, non-zero flags, and the reference's
+    checksums (test_3c2e.py:303-319, 10 places); the Cartesian twins over all
+    eight shells against the oracle."""
+    _say("fixture", name)
+    B = mc.Basis(*fixture_basis)
+    if name.startswith("int2c2e"):
+        q = np.array([(i, k, 0, 0) for k in range(4) for i in range(4)], np.int32)
+        qc = np.array([(i, k, 0, 0) for k in range(8) for i in range(8)], np.int32)
+    else:
+        q = np.array([(i, j, k, 0) for k in range(4) for j in range(4) for i in range(4)], np.int32)
+        qc = np.array([(i, j, k, 0) for k in range(8) for j in range(8) for i in range(8)], np.int32)
+    got, off, nz = mc.batch(name, q, B)
+    ref_o, off_r, nz_r = orc.batch(name, q, orc.Basis(*fixture_basis))
+    np.testing.assert_array_equal(off, off_r)
+    _cmp(got, ref_o, name)
+    np.testing.assert_array_equal(nz, nz_r)
+    v = float(np.abs(got).sum())
+    assert close_place(v, ref, got.size, 10), (name, v, ref)
+    cart = name.replace("_sph", "_cart")
+    got, _, nz = mc.batch(cart, qc, B)
+    ref_o, _, nz_r = orc.batch(cart, qc, orc.Basis(*fixture_basis))
+    _cmp(got, ref_o, cart)
+    np.testing.assert_array_equal(nz, nz_r)
+
+
+@pytest.mark.parametrize("omega", [0.0, 0.3, -0.3])
+def test_c4_3c2e_derivative_and_2c2e_samples(mc, orc, omega):
+    """The 40-atom cluster's orbital and auxiliary shells: int3c2e_ip1 /
+    int3c2e_ip2 on (i, j | P) triples and int2c2e on (P|Q) pairs of the
+    auxiliary basis, plain, long-range erf and short-range erfc."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c4")
+    env = m.env.copy()
+    env[8] = omega
+    rng = np.random.default_rng(44)
+    n = 300
+    q = np.zeros((n, 4), np.int32)
+    q[:, 0] = rng.integers(0, m.aux_start, n)
+    q[:, 1] = rng.integers(0, m.aux_start, n)
+    q[:, 2] = rng.integers(m.aux_start, m.nbas, n)
+    q2 = np.zeros((n, 4), np.int32)
+    q2[:, 0] = rng.integers(m.aux_start, m.nbas, n)
+    q2[:, 1] = rng.integers(m.aux_start, m.nbas, n)
+    _say("c4 3c derivative / 2c samples, omega", omega)
+    orc.set_root_mode(orc.ROOTS_TABLE)
+    try:
+        for name, qq in (("int3c2e_ip1_sph", q), ("int3c2e_ip2_sph", q), ("int2c2e_sph", q2),
+                         ("int3c2e_ip2_cart", q[:100]), ("int2c2e_cart", q2[:100])):
+            got, _, nz = mc.batch(name, qq, mc.Basis(m.atm, m.bas, env))
+            ref, _, nz_r = orc.batch(name, qq, orc.Basis(m.atm, m.bas, env))
+            _cmp(got, ref, f"c4 {name} omega {omega}")
+            np.testing.assert_array_equal(nz, nz_r)
+    finally:
+        orc.set_root_mode(orc.ROOTS_EXACT)
 
 
 def test_batch_with_caller_offsets(mc, fixture_basis):
@@ -142,7 +223,7 @@ def test_synthetic_samples(mc, orc, cfg, n):
     from libcint_amd.synthetic import make_molecule
     m = make_molecule(cfg)
     nb = m.nbas if m.aux_start < 0 else m.aux_start
-    q = _sample_quartets(nb, n, seed=hash(cfg) % 1000)
+    q = _sample_quartets(nb, n, seed=zlib.crc32(cfg.encode()) % 1000)
     _say(cfg, "sample of", n, "quartets")
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
@@ -200,6 +281,62 @@ def test_per_call_with_dims(mc, orc, fixture_basis):
     o = out.reshape(dims, order="F")
     _cmp(o[: d[0], : d[1], : d[2], : d[3]], ref, "dims block")
     assert np.all(o[d[0]:] == 7.0) and np.all(o[:, d[1]:] == 7.0)
+
+
+def test_per_call_threads_share_optimizer(mc, fixture_basis):
+    """libcint's documented usage (examples/time_c2h6.c:719-745): one
+    int2e_optimizer shared by 8 host threads, each calling int2e_sph /
+    int2e_ip1_sph per shell tuple.  Every block equals the batch entry
+    point's; the optimizer holds the device state (basis, all pair data), so
+    a call with it only launches and copies.  Per-call latency with and
+    without the optimizer is printed."""
+    import ctypes
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+    L = mc.lib()
+    B = mc.Basis(*fixture_basis)
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    args = (B.atm.ctypes.data_as(ip), B.natm, B.bas.ctypes.data_as(ip), B.nbas,
+            B.env.ctypes.data_as(dp))
+    for name in ("int2e_sph", "int2e_ip1_sph"):
+        q = libcint_quartets()
+        ref, off, nz_ref = mc.batch(name, q, B)
+        opt = ctypes.c_void_p()
+        getattr(L, name.replace("_sph", "") + "_optimizer")(ctypes.byref(opt), *args)
+        assert L.mcint_optimizer_on_device(opt) == 1
+        got = np.full(ref.size, np.nan)
+        st = np.full(q.shape[0], -1, dtype=np.int32)
+        f = getattr(L, name)
+        shl = np.ascontiguousarray(q, dtype=np.int32)
+
+        def work(rows):
+            for n in rows:
+                dst = got[off[n]:off[n + 1]]
+                st[n] = f(dst.ctypes.data_as(dp), None, shl[n].ctypes.data_as(ip), args[0], args[1],
+                          args[2], args[3], args[4], opt, None)
+
+        _say(name, "per call from 8 threads sharing one optimizer,", q.shape[0], "tuples")
+        with ThreadPoolExecutor(8) as ex:
+            list(ex.map(work, [range(t, q.shape[0], 8) for t in range(8)]))
+        np.testing.assert_array_equal(st, nz_ref)
+        _cmp(got, ref, f"{name} per call with a shared optimizer")
+        # latency, one thread: with the optimizer, and without (upload + plan)
+        rows = range(0, q.shape[0], max(1, q.shape[0] // 200))
+        t0 = time.perf_counter()
+        work(rows)
+        t_opt = (time.perf_counter() - t0) / len(rows)
+        t0 = time.perf_counter()
+        for n in rows:
+            dst = got[off[n]:off[n + 1]]
+            f(dst.ctypes.data_as(dp), None, shl[n].ctypes.data_as(ip), args[0], args[1], args[2],
+              args[3], args[4], None, None)
+        t_none = (time.perf_counter() - t0) / len(rows)
+        _say(f"{name} per-call latency: {t_opt * 1e6:.0f} us with the optimizer, "
+             f"{t_none * 1e6:.0f} us without")
+        _cmp(got, ref, f"{name} per call without an optimizer")
+        L.CINTdel_optimizer(ctypes.byref(opt))
+        assert not opt.value
 
 
 def test_screened_quartet_is_zero_filled(mc, orc):
@@ -382,17 +519,26 @@ def _ip1_translation_residual(batch, q, bas):
     return res
 
 
+# Share of the quartets of one ip1 comparison that may use the allowance
+# of _cmp_ip1 (at least two quartets).
+IP1_ALLOWANCE_FRAC = 0.02
+
+
 def _cmp_ip1(mc, orc, m, q, got, ref, off, what):
     """ip1 parity: every element within the bar, except in quartets where the
     reference's own result is off by more than that (its translational
     invariance residual exceeds the bar); there the GPU result must satisfy
     invariance much better than the reference and differ from it by no more
-    than the reference's residual."""
+    than the reference's residual.  At most IP1_ALLOWANCE_FRAC of the
+    quartets (and at least two) may take the allowance."""
     err = np.abs(got - ref)
     bad = err > ATOL + RTOL * np.abs(ref)
     if not bad.any():
+        _say(what, "ip1: every element within the bar")
         return
     qi = np.unique(np.searchsorted(off, np.nonzero(bad)[0], side="right") - 1)
+    _say(what, "ip1 allowance taken by", len(qi), "of", len(q), "quartets")
+    assert len(qi) <= max(2, IP1_ALLOWANCE_FRAC * len(q)), (what, len(qi), len(q))
     qb = q[qi]
     r_ref = _ip1_translation_residual(
         lambda n, qq: orc.batch(n, qq, orc.Basis(m.atm, m.bas, m.env)), qb, m.bas)
