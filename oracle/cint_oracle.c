@@ -531,10 +531,10 @@ static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas
 }
 
 /* Three-centre (ij|k), src/g3c2e.c:21-132; deriv_i adds the derivative
- * on i (int3c2e_ip1, ng = {1,0,0,0,...}, src/autocode/int3c2e.c:80-87),
+ * on i (int3c2e_ip1, ng = {1,0,0,0,...}, src/autocode/int3c2e.c:72-87),
  * deriv_k the derivative on the auxiliary shell k (int3c2e_ip2, ng =
- * {0,0,1,0,...}, :161-168).  two_center: (i|k) of int2c2e
- * (src/g2c2e.c:15-90, src/cint2c2e.c:351-389), the j slot an exponent-0 s
+ * {0,0,1,0,...}, :153-168).  two_center: (i|k) of int2c2e
+ * (src/g2c2e.c:15-90, src/cint2c2e.c:351-374), the j slot an exponent-0 s
  * function without an angular factor. */
 static void oc_init_3c2e(OcEnv *e, const int *shls, const int *atm, const int *bas,
                          const double *env, int deriv_i, int deriv_k, int two_center)
@@ -1169,8 +1169,8 @@ int orc_int3c2e_cart(double *out, const int *dims, const int *shls, const int *a
         return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 1, 0);
 }
 
-/* int3c2e_ip1 (src/autocode/int3c2e.c:80-95), int3c2e_ip2 (:161-176) and
- * int2c2e (src/cint2c2e.c:351-389; shls = (i, k)). */
+/* int3c2e_ip1 (src/autocode/int3c2e.c:72-87), int3c2e_ip2 (:153-168) and
+ * int2c2e (src/cint2c2e.c:351-374; shls = (i, k)). */
 int orc_int3c2e_ip1_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
                         const int *bas, int nbas, const double *env)
 {
