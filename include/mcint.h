@@ -249,6 +249,19 @@ int64_t mcint_plan_fp32_tasks(const mcint_plan *p);
 int mcint_plan_execute(mcint_plan *p, double *out_dev, FINT *not0_dev, void *hip_stream);
 /* Same, launching the kernels directly (no graph). */
 int mcint_plan_execute_direct(mcint_plan *p, double *out_dev, FINT *not0_dev, void *hip_stream);
+/* No counterpart in libcint.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops): as
+ * mcint_plan_execute_direct, but the chunked and generic paths' per-task
+ * blocks and flags live in a fixed device array of
+ * MCINT_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB per GPU, held until the
+ * process ends) instead of the plan's lazily allocated scratch; buckets run
+ * in slices that fit it.  Calls no hipMalloc/hipFree.  Returns
+ * MCINT_ERR_UNSUPPORTED, before any launch, when one task of some class
+ * needs more block space than the array holds.  Calls that overlap on one
+ * GPU share the array and race. */
+#define MCINT_NOALLOC_SCRATCH_DOUBLES (8ll << 20)
+int mcint_plan_execute_noalloc(mcint_plan *p, double *out_dev, FINT *not0_dev, void *hip_stream);
+int64_t mcint_noalloc_scratch_doubles(void);
 void mcint_plan_destroy(mcint_plan *p);
 
 /* Plan statistics: n[0] tasks, n[1] buckets, n[2] tasks on fast kernels,

@@ -86,6 +86,10 @@ def lib():
     L.mcint_plan_execute.argtypes = [vp, vp, vp, vp]
     L.mcint_plan_execute_direct.restype = ctypes.c_int
     L.mcint_plan_execute_direct.argtypes = [vp, vp, vp, vp]
+    L.mcint_plan_execute_noalloc.restype = ctypes.c_int
+    L.mcint_plan_execute_noalloc.argtypes = [vp, vp, vp, vp]
+    L.mcint_noalloc_scratch_doubles.restype = ctypes.c_int64
+    L.mcint_noalloc_scratch_doubles.argtypes = []
     L.mcint_plan_destroy.argtypes = [vp]
     L.mcint_plan_stats.restype = ctypes.c_int
     L.mcint_plan_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
@@ -343,6 +347,15 @@ class Plan:
         f = lib().mcint_plan_execute if graph else lib().mcint_plan_execute_direct
         _check(f(self.handle, ctypes.c_void_p(out_ptr), ctypes.c_void_p(not0_ptr or None),
                  ctypes.c_void_p(stream_ptr or None)), "mcint_plan_execute")
+
+    def execute_noalloc(self, out_ptr: int, not0_ptr: int = 0, stream_ptr: int = 0):
+        """As execute(graph=False), allocating no device memory: per-task
+        blocks live in the library's fixed device array (calls that overlap
+        on one GPU race)."""
+        _check(lib().mcint_plan_execute_noalloc(self.handle, ctypes.c_void_p(out_ptr),
+                                                ctypes.c_void_p(not0_ptr or None),
+                                                ctypes.c_void_p(stream_ptr or None)),
+               "mcint_plan_execute_noalloc")
 
     def execute_grad(self, dm_ptr: int, nao: int, grad_ptr: int, stream_ptr: int = 0):
         """Gradient digest of an int2e_ip1_sph plan: grad (natm x 3, device)

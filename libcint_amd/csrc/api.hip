@@ -1336,6 +1336,8 @@ int mcint::plan_launch_scratch(mcint_plan* p, double* out, FINT* not0, hipStream
   return plan_launch(p, out, not0, stream, S);
 }
 
+int64_t mcint::plan_tasks(const mcint_plan* p) { return p->ntasks; }
+
 void mcint::set_error(const char* msg) { set_err(msg); }
 
 // ---------------------------------------------------------------------------

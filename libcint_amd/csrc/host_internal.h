@@ -14,6 +14,9 @@ namespace mcint {
 int plan_launch_scratch(mcint_plan* p, double* out, FINT* not0, hipStream_t stream,
                         double* blocks, int64_t nblocks, int* flags, int64_t nflags);
 
+// Tasks (contracted quartets after the virtual-shell expansion) of plan p.
+int64_t plan_tasks(const mcint_plan* p);
+
 // Sets the calling thread's mcint_last_error() text.
 void set_error(const char* msg);
 
