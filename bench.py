@@ -446,7 +446,7 @@ def main():
     except (OSError, KeyError, ValueError):
         traffic = None
     if args.bucket_table:
-        rows = [dict(bucket=i, cls=list(b["cls"]), perm=b["perm"], team=b["team"], path=b["path"],
+        rows = [dict(bucket=i, cls=list(b["cls"]), team=b["team"], path=b["path"],
                      tasks=b["tasks"], prim_quartets=int(nprim[i]), ms=btime[i] * 1e3,
                      model_flops=bflops[i], tflops=bflops[i] / btime[i] / 1e12,
                      frac=bflops[i] / btime[i] / 1e12 / FP64_PEAK_TFLOPS)

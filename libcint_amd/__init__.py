@@ -311,7 +311,8 @@ class Plan:
                     fp32_tasks=int(lib().mcint_plan_fp32_tasks(self.handle)))
 
     def buckets(self) -> list:
-        """Per bucket: dict(cls=(la,lb,lc,ld), perm, team, path, tasks)."""
+        """Per bucket: dict(cls=(la,lb,lc,ld), perm (-1: every axis order of the class), team,
+        path, tasks)."""
         L = lib()
         out = []
         info = (ctypes.c_int64 * 8)()

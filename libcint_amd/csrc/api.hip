@@ -6,7 +6,9 @@
 //      contracted shells into virtual shells;
 //   2. mcint_plan_create canonicalises every shell tuple (permutational
 //      symmetry of (ij|kl)), sorts the tuples into buckets of equal
-//      (class, permutation, team size), and builds the primitive-pair data
+//      (class, team size, path) -- every axis order of a class shares a
+//      bucket, each task carrying its own axis map -- and builds the
+//      primitive-pair data
 //      of every pair the tuples touch (the role of int2e_optimizer,
 //      src/optimizer.c:344-425);
 //   3. mcint_plan_execute launches one kernel per bucket, captured once into
@@ -14,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -333,10 +336,10 @@ void tuple_class(int kind, const QuartetShape& Q, int& la, int& lb, int& lc, int
 }
 
 // The tasks of one described tuple s: one per combination of contraction
-// columns, in canonical shell order, output offsets from base.
+// columns, in canonical shell order, output offsets from base, axis map perm.
 template <class F>
 void for_each_task(const std::vector<int>& vfirst, int dummy, int kind, const QuartetShape& Q,
-                   const FINT* s, int64_t base, F&& emit) {
+                   const FINT* s, int64_t base, int perm, F&& emit) {
   const int naxes = kind_axes(kind);
   int cidx[4] = {0, 0, 0, 0};
   for (;;) {
@@ -357,7 +360,7 @@ void for_each_task(const std::vector<int>& vfirst, int dummy, int kind, const Qu
         }
       }
     }
-    t.out = o;
+    t.outp = task_pack(o, perm);
     emit(t);
     int a = 0;
     for (; a < naxes; ++a) {
@@ -421,7 +424,7 @@ struct KeyIndex {
 };
 
 struct Bucket {
-  int la, lb, lc, ld, perm, team;
+  int la, lb, lc, ld, team;
   int mode;  // 0 generic, 1 fast (single chunk), 2 chunked
   int fp32;  // recurrences in FP32 (mode 1 only)
   int64_t t0, t1;
@@ -630,9 +633,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
                                 : 0u;
       int tl = 0;
       while ((1 << tl) < team) ++tl;
+      // the axis map (perm) travels with each task, not in the key
       qkey[q] = (uint32_t)la | ((uint32_t)lb << 3) | ((uint32_t)lc << 6) | ((uint32_t)ld << 9) |
-                ((uint32_t)perm << 12) | ((uint32_t)tl << 20) | ((uint32_t)mode << 24) |
-                (lowp << 26);
+                ((uint32_t)tl << 20) | ((uint32_t)mode << 24) | (lowp << 26);
       uint32_t nt = 1;
       for (int a = 0; a < naxes; ++a) nt *= (uint32_t)Q.nc[a];
       qnt[q] = nt;
@@ -735,10 +738,12 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     for (size_t q = q0; q < q1; ++q) {
       if (!qnt[q]) continue;
       describe(q, Q);
+      int la, lb, lc, ld, perm;
+      tuple_class(kind, Q, la, lb, lc, ld, perm);
       const FINT* s = shls + 4 * q;
       int64_t& d = cur[qkey[q]];
       // virtual expansion over contraction columns of the original axes
-      for_each_task(b->vfirst, b->dummy, kind, Q, s, qoff[q], [&](const Task& t) {
+      for_each_task(b->vfirst, b->dummy, kind, Q, s, qoff[q], perm, [&](const Task& t) {
         sorted[d] = t;
         sorted_q[d] = (int)q;
         ++d;
@@ -746,6 +751,48 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     }
   });
   timer.mark("task placement");
+  // Buckets of one- and two-lane teams hold tasks of 1 to 2*ppl primitive
+  // quartets; a wave runs as long as its longest task, so these buckets are
+  // ordered by descending primitive-quartet count (stable counting sort:
+  // list order within equal counts).  Larger teams split a task's
+  // primitives over their lanes and keep the list order.
+  {
+    const int naxes_s = kind_axes(kind);
+    std::vector<size_t> small;
+    for (size_t i = 0; i < nkeys; ++i)
+      if (((ukeys[i] >> 20) & 15) <= 1 && ((ukeys[i] >> 24) & 3) != 0 && cnt[i + 1] - cnt[i] > 1)
+        small.push_back(i);
+    std::atomic<size_t> next{0};
+    const int nw = std::min<int>(nthr, (int)small.size());
+    parallel_for(nw, (size_t)nw, [&](int, size_t, size_t) {
+      constexpr int KMAX = 256;
+      std::vector<Task> tt;
+      std::vector<int> tq;
+      std::vector<uint16_t> key;
+      for (size_t w; (w = next.fetch_add(1)) < small.size();) {
+        const size_t i = small[w];
+        const int64_t a = cnt[i], n = cnt[i + 1] - cnt[i];
+        key.resize(n);
+        int64_t hist[KMAX + 1] = {};
+        for (int64_t t = 0; t < n; ++t) {
+          const Task& tk = sorted[a + t];
+          int64_t np = 1;
+          for (int x = 0; x < naxes_s; ++x) np *= b->vsh[tk.sh[x]].nprim;
+          key[t] = (uint16_t)(KMAX - std::min<int64_t>(np, KMAX));  // descending
+          ++hist[key[t] + 1];
+        }
+        for (int k = 0; k < KMAX; ++k) hist[k + 1] += hist[k];
+        tt.assign(sorted.get() + a, sorted.get() + a + n);
+        tq.assign(sorted_q.get() + a, sorted_q.get() + a + n);
+        for (int64_t t = 0; t < n; ++t) {
+          const int64_t d = hist[key[t]]++;
+          sorted[a + d] = tt[t];
+          sorted_q[a + d] = tq[t];
+        }
+      }
+    });
+  }
+  timer.mark("order small-team buckets by primitive count");
   int64_t max_gen = 0, max_flags = 0;
   for (size_t i = 0; i < nkeys; ++i) {
     const uint32_t k = ukeys[i];
@@ -754,7 +801,6 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     bk.lb = (k >> 3) & 7;
     bk.lc = (k >> 6) & 7;
     bk.ld = (k >> 9) & 7;
-    bk.perm = (k >> 12) & 255;
     bk.team = 1 << ((k >> 20) & 15);
     bk.mode = (k >> 24) & 3;
     bk.fp32 = (k >> 26) & 1;
@@ -906,7 +952,6 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.ld = bk.ld;
   A.kind = p->kind;
   A.sph = p->sph;
-  A.perm = bk.perm;
   A.team = bk.team;
   A.fp32 = bk.fp32;
   A.t0 = bk.t0;
@@ -1214,7 +1259,8 @@ extern "C" int mcint_plan_bucket_info(const mcint_plan* p, int i, int64_t* info)
     return MCINT_ERR_ARG;
   }
   const Bucket& b = p->buckets[i];
-  const int64_t v[8] = {b.la, b.lb, b.lc, b.ld, b.perm, b.team, b.mode, b.t1 - b.t0};
+  // perm -1: the bucket holds every axis order of its class
+  const int64_t v[8] = {b.la, b.lb, b.lc, b.ld, -1, b.team, b.mode, b.t1 - b.t0};
   for (int k = 0; k < 8; ++k) info[k] = v[k];
   return MCINT_OK;
 }
@@ -1512,7 +1558,9 @@ hipError_t ctx_reserve(CallCtx* c, int nt, int64_t blk, int64_t nscr, int nflag)
     e = hipHostMalloc((void**)&c->task_h, sizeof(Task) * nt);
     if (e == hipSuccess) e = dalloc(&c->task_d, nt);
     if (e == hipSuccess) e = dalloc(&c->tid_d, nt);
-    if (e == hipSuccess) e = hipMemset(c->tid_d, 0, sizeof(int) * nt);
+    // on the context's own stream: its kernels run on that non-blocking
+    // stream, which does not wait for null-stream work, and read the ids
+    if (e == hipSuccess) e = hipMemsetAsync(c->tid_d, 0, sizeof(int) * nt, c->stream);
     if (e == hipSuccess) c->ntask_cap = nt;
   }
   if (e == hipSuccess && blk > c->out_cap) {
@@ -1704,7 +1752,8 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
   // virtual tasks over the contraction columns, as the plan's placement
   {
     int t = 0;
-    for_each_task(b->vfirst, b->dummy, kind, Q, shls, 0, [&](const Task& tk) { c->task_h[t++] = tk; });
+    for_each_task(b->vfirst, b->dummy, kind, Q, shls, 0, perm,
+                  [&](const Task& tk) { c->task_h[t++] = tk; });
   }
   BasisView B;
   B.vsh = b->vsh_d;
@@ -1733,7 +1782,6 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
   A.ld = ld;
   A.kind = kind;
   A.sph = sph;
-  A.perm = perm;
   A.team = team;
   A.fp32 = 0;
   A.dm = nullptr;

@@ -179,8 +179,8 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
   VShell s3 = s2;
   if (!three) s3 = B.vsh[tk.sh[3]];
-  const OutStrides OS = out_strides(B.vsh, tk, A.perm, three ? 3 : 4, A.sph);
-  double* out = A.out + tk.out;
+  const OutStrides OS = out_strides(B.vsh, tk, task_perm(tk), three ? 3 : 4, A.sph);
+  double* out = A.out + task_out(tk);
 
   // component tables of the union e-list and the f-list
   for (int i = lane; i < G.nE; i += GEN_LANES) {

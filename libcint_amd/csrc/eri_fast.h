@@ -430,11 +430,11 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
 }
 
 // Output strides of the canonical axes: axis a holds the original axis
-// (perm >> 2a) & 3 of the packed Fortran block.
+// (perm >> 2a) & 3 of the packed Fortran block (perm: the task's axis map).
 template <bool SPH = true>
-__device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& tk, int perm,
-                                              int64_t& st0, int64_t& st1, int64_t& st2,
-                                              int64_t& st3) {
+__device__ __forceinline__ void canon_strides(const BasisView& Bv, const Task& tk, int64_t& st0,
+                                              int64_t& st1, int64_t& st2, int64_t& st3) {
+  const int perm = task_perm(tk);
   int dorig[4];
   dorig[perm & 3] = (SPH ? Bv.vsh[tk.sh[0]].dsph : Bv.vsh[tk.sh[0]].dcart);
   dorig[(perm >> 2) & 3] = (SPH ? Bv.vsh[tk.sh[1]].dsph : Bv.vsh[tk.sh[1]].dcart);
@@ -511,8 +511,8 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
 
   // ---- epilogue --------------------------------------------------------
   int64_t st0, st1, st2, st3;
-  canon_strides<SPH>(Bv, tk, A.perm, st0, st1, st2, st3);
-  double* __restrict__ out = A.out + tk.out;
+  canon_strides<SPH>(Bv, tk, st0, st1, st2, st3);
+  double* __restrict__ out = A.out + task_out(tk);
 
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
   constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
@@ -791,7 +791,7 @@ __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
 // Ket transfer and spherical c, d of the rows r0, r0+rs, ... of Ys, then
 // the store with the task's output strides; any == 0 writes zeros.
 template <int LA, int LB, int LC, int LD, bool SPH = true>
-__device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& tk, int perm,
+__device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& tk,
                                               const double* __restrict__ Ys,
                                               double* __restrict__ outbase, int any, int r0,
                                               int rs) {
@@ -799,8 +799,8 @@ __device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& t
   constexpr int NF = C::NF, NC = C::NC, ND = C::ND;
   constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
   int64_t st0, st1, st2, st3;
-  canon_strides<SPH>(Bv, tk, perm, st0, st1, st2, st3);
-  double* __restrict__ out = outbase + tk.out;
+  canon_strides<SPH>(Bv, tk, st0, st1, st2, st3);
+  double* __restrict__ out = outbase + task_out(tk);
   if (!any) {
     for (int k = r0; k < NAo * NBo * NCo * NDo; k += rs) {
       const int ao = k % NAo, bo = (k / NAo) % NBo, co = (k / (NAo * NBo)) % NCo;
@@ -879,7 +879,7 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
   __syncthreads();
   if (valid && any) epi_bra<LA, LB, LC, LD, SPH>(Bv, tk, X, Ys, tl, TE);
   __syncthreads();
-  if (valid) epi_ket_store<LA, LB, LC, LD, SPH>(Bv, tk, A.perm, Ys, A.out, any, tl, TE);
+  if (valid) epi_ket_store<LA, LB, LC, LD, SPH>(Bv, tk, Ys, A.out, any, tl, TE);
 }
 
 // ---------------------------------------------------------------------
@@ -1029,10 +1029,10 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
     const VShell& sc = Bv.vsh[tk.sh[2]];
     const VShell& sd = Bv.vsh[tk.sh[3]];
     int64_t st0, st1, st2, st3;
-    canon_strides<SPH>(Bv, tk, A.perm, st0, st1, st2, st3);
+    canon_strides<SPH>(Bv, tk, st0, st1, st2, st3);
     const int64_t stc = SPH ? (int64_t)sa.dsph * sb.dsph * sc.dsph * sd.dsph
                             : (int64_t)sa.dcart * sb.dcart * sc.dcart * sd.dcart;
-    double* __restrict__ out = A.out + tk.out;
+    double* __restrict__ out = A.out + task_out(tk);
     if (!any) {
       if (!digest)
         for (int k = tl; k < 3 * NAo * NBo * NCo * NDo; k += TE) {

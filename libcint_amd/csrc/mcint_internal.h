@@ -72,10 +72,20 @@ struct PairPrim {
 };
 
 // A unit of work: one contracted (virtual) quartet in canonical shell order.
+// outp packs the offset of element (0,0,0,0) of the quartet's block (low 56
+// bits) and the axis map (high 8 bits): canonical axis A holds the original
+// axis (perm >> 2A) & 3, so tasks of every axis order of one class share a
+// launch.
 struct Task {
   int16_t sh[4];   // virtual shells, canonical order (sh[3] unused for 3c2e)
-  int64_t out;     // offset of element (0,0,0,0) of this quartet's block
+  int64_t outp;
 };
+constexpr int64_t TASK_OUT_MASK = (int64_t(1) << 56) - 1;
+__host__ __device__ inline int64_t task_out(const Task& t) { return t.outp & TASK_OUT_MASK; }
+__host__ __device__ inline int task_perm(const Task& t) { return (int)((uint64_t)t.outp >> 56); }
+__host__ __device__ inline int64_t task_pack(int64_t out, int perm) {
+  return out | ((int64_t)perm << 56);
+}
 
 // Device view of a basis (atm/bas/env) plus its pair data.
 struct BasisView {
@@ -92,7 +102,7 @@ struct BasisView {
 };
 
 // Launch description of one bucket: tasks [t0, t1) share class, kind,
-// permutation and team size.
+// team size and path (each task carries its own axis map).
 struct BucketArgs {
   const Task* tasks;
   int64_t t0, t1;
@@ -105,7 +115,6 @@ struct BucketArgs {
   int la, lb, lc, ld;    // canonical angular momenta
   int kind;
   int sph;
-  int perm;              // canonical axis A maps to original axis (perm >> 2A) & 3
   int team;              // lanes per task (fast kernels)
   int fp32;              // recurrences and contraction in FP32 (fast kernels)
   // gradient digest (i-derivative plans): instead of storing the block,

@@ -1,8 +1,8 @@
 """Full-size GPU checks on the synthetic workloads.
 
 * Every bucket of the real plans: at least four quartets (tasks picked at
-  random positions) from each (class, axis permutation, team size, kernel
-  path) bucket of the c1 int2e_sph plan (205M Schwarz-surviving quartets of
+  random positions) from each (class, team size, kernel path) bucket and
+  each original angular-momentum order (axis permutation) inside it of the c1 int2e_sph plan (205M Schwarz-surviving quartets of
   the 30-atom cluster), of the c3 ip1 gradient list, the c3 int2e_cart list
   and the c4 int3c2e list, checked element by element against the oracle.
 * c2: the mixed-precision J/K build on the 30-atom cluster, FP32 tier for
@@ -39,33 +39,52 @@ def mc():
 
 
 def _bucket_keys(plan):
-    return sorted({(b["cls"], b["perm"], b["team"], b["path"]) for b in plan.buckets()})
+    return sorted({(b["cls"], b["team"], b["path"]) for b in plan.buckets()})
 
 
-def _stratified_sample(plan, q, per_bucket, seed):
-    """per_bucket tasks at random positions of every bucket (all of a
-    smaller bucket), as rows of the plan's list q."""
+def _order_code(bas, q):
+    """The angular momenta of each tuple in its original order, packed: the
+    axis map that canonicalises a tuple (the per-task permutation inside a
+    bucket) is a function of this order."""
+    code = np.zeros(len(q), np.int64)
+    for a in range(4):
+        code = code * 8 + bas[q[:, a], 1]
+    return code
+
+
+def _stratified_sample(plan, q, bas, per_group, seed):
+    """per_group tasks at random positions of every (bucket, original
+    angular-momentum order) group (all of a smaller group), as rows of the
+    plan's list q; also the set of groups."""
     rng = np.random.default_rng(seed)
-    picks = []
+    picks, groups = [], set()
     for i, b in enumerate(plan.buckets()):
-        n = int(b["tasks"])
-        for p0 in np.sort(rng.choice(n, size=min(per_bucket, n), replace=False)):
-            picks.append(int(plan.bucket_tasks(i, int(p0), 1)[0]))
-    return q[np.unique(np.array(picks, dtype=np.int64))]
+        ids = plan.bucket_tasks(i, 0, int(b["tasks"])).astype(np.int64)
+        code = _order_code(bas, q[ids])
+        for c in np.unique(code):
+            sel = ids[code == c]
+            groups.add((b["cls"], b["team"], b["path"], int(c)))
+            picks.append(rng.choice(sel, size=min(per_group, sel.size), replace=False))
+    return q[np.unique(np.concatenate(picks))], groups
 
 
 def _check_stratified(mc, orc, m, name, q, per_bucket=4, seed=0):
     db = mc.DeviceBasis(mc.Basis(m.atm, m.bas, m.env))
     plan = mc.Plan(db, name, q)
     keys = _bucket_keys(plan)
-    sample = _stratified_sample(plan, q, per_bucket, seed)
+    sample, groups = _stratified_sample(plan, q, m.bas, per_bucket, seed)
     plan.close()
     splan = mc.Plan(db, name, sample)
     skeys = _bucket_keys(splan)
+    sgroups = {(b["cls"], b["team"], b["path"], int(c))
+               for i, b in enumerate(splan.buckets())
+               for c in np.unique(_order_code(m.bas, sample[splan.bucket_tasks(i, 0, int(b["tasks"]))]))}
     splan.close()
     db.close()
-    _say(m.name, name, len(q), "quartets,", len(keys), "buckets; sample of", len(sample))
+    _say(m.name, name, len(q), "quartets,", len(keys), "buckets,", len(groups),
+         "(bucket, axis order) groups; sample of", len(sample))
     assert skeys == keys, "the sample's plan must hold every bucket of the full plan"
+    assert sgroups == groups, "the sample must hold every axis order of every bucket"
     got, off, nz = mc.batch(name, sample, mc.Basis(m.atm, m.bas, m.env))
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
