@@ -349,6 +349,9 @@ def main():
                     help="skip the J/K, nuclear-gradient and three-centre legs")
     ap.add_argument("--bucket-table", default="",
                     help="write every bucket's class, path, tasks, time and FLOP rate (JSON)")
+    ap.add_argument("--only-buckets", default="",
+                    help="run only the buckets of these classes, e.g. '3221:1,3321:1' "
+                         "(la lb lc ld : team), --steps times each, then exit (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -391,6 +394,28 @@ def main():
     log(f"output {eris * 8 / 1e9:.1f} GB")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
+    if args.only_buckets:
+        want = {(tuple(int(c) for c in k.split(":")[0]), int(k.split(":")[1]))
+                for k in args.only_buckets.split(",")}
+        bl = plan.buckets()
+        nprim = plan.count_primitives()
+        for i, b in enumerate(bl):
+            if (tuple(b["cls"]), b["team"]) not in want:
+                continue
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            plan.execute_bucket(i, out.data_ptr(), stream_ptr=sp)
+            e0.record(stream)
+            for _ in range(args.steps):
+                plan.execute_bucket(i, out.data_ptr(), stream_ptr=sp)
+            e1.record(stream)
+            e1.synchronize()
+            t = e0.elapsed_time(e1) * 1e-3 / args.steps
+            fl = bucket_flops(b["cls"], nprim[i], b["tasks"])
+            print(json.dumps(dict(bucket=i, cls=list(b["cls"]), team=b["team"], path=b["path"],
+                                  tasks=b["tasks"], prim_quartets=int(nprim[i]), ms=t * 1e3,
+                                  model_flops=fl, frac=fl / t / 1e12 / FP64_PEAK_TFLOPS)), flush=True)
+        return
 
     for _ in range(args.warmup):
         plan.execute(out.data_ptr(), stream_ptr=sp)

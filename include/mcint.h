@@ -333,11 +333,15 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * afterwards to the generic kernel (runtime class, one workgroup per
  * quartet, no register kernels): the plain path that the benchmark times
  * and cross-checks the register kernels against; 0 (default) restores the
- * class-specialised kernels.
+ * class-specialised kernels.  MCINT_TUNE_GRAPH_BRANCHES (1..8, default
+ * 4): mcint_plan_execute on a stream captures the plan's buckets into that
+ * many concurrent graph branches (buckets balanced by estimated work, one
+ * scratch region per branch); takes effect at the next capture.
  * Applies to plans created afterwards; returns the previous value, or a
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0
 #define MCINT_TUNE_GENERIC_ONLY 1
+#define MCINT_TUNE_GRAPH_BRANCHES 2
 int mcint_tune(int key, int value);
 /* Per-class override of MCINT_TUNE_PRIMS_PER_LANE for one integral kind
  * (intor code) and canonical class (la >= lb, lc >= ld, (la,lb) >= (lc,ld)

@@ -429,7 +429,28 @@ struct Bucket {
   int fp32;  // recurrences in FP32 (mode 1 only)
   int64_t t0, t1;
   int64_t sstride;  // generic scratch doubles per task
+  double cost;      // estimated work (balances the concurrent graph branches)
 };
+
+// Cartesian components of shells lo..hi
+constexpr int ncart_sum(int lo, int hi) {
+  int s = 0;
+  for (int l = lo; l <= hi; ++l) s += (l + 1) * (l + 2) / 2;
+  return s;
+}
+
+// Estimated work of a bucket: per primitive quartet and root the 2D
+// recurrence and the [e0|f0] products, per task the transfers; the generic
+// kernel counted 15x (its measured slowdown).
+double bucket_cost(int kind, int mode, int la, int lb, int lc, int ld, int64_t nprim,
+                   int64_t ntask) {
+  const int d = kind_ncomp(kind) == 3 ? 1 : 0;
+  const int E = la + lb + d, F = lc + ld;
+  const int nr = (E + F) / 2 + 1;
+  const double nef = (double)ncart_sum(la, E) * ncart_sum(lc, F);
+  const double c = (double)nprim * nr * (2.0 * nef + 9.0 * (E + 1) * (F + 1)) + 4.0 * ntask * nef;
+  return mode == 0 ? 15.0 * c : c;
+}
 
 // Scratch bound of the generic and chunked paths (doubles); buckets larger
 // than the scratch are launched in slices.
@@ -442,6 +463,9 @@ constexpr int GRAD_SLOTS = 64;
 int g_prims_per_lane = 32;
 // 1: every quartet on the generic kernel (mcint_tune MCINT_TUNE_GENERIC_ONLY)
 int g_generic_only = 0;
+// concurrent branches of a captured plan graph (mcint_tune MCINT_TUNE_GRAPH_BRANCHES)
+int g_graph_branches = 4;
+constexpr int MAX_GRAPH_BRANCHES = 8;
 // per-class overrides (mcint_tune_class), 0 = the global value; indexed by
 // integral kind and canonical class
 constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
@@ -480,18 +504,27 @@ struct mcint_plan {
   double* gslots_d = nullptr;  // gradient digest accumulators
   int64_t stats[6] = {0, 0, 0, 0, 0, 0};
   int64_t stats_fp32 = 0;
+  // per-branch scratch: scratch_d / flags_d hold scratch_nbr regions of
+  // scratch_size doubles / flags_size ints
+  int scratch_nbr = 0;
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double* g_out = nullptr;
   FINT* g_not0 = nullptr;
   hipStream_t g_stream = nullptr;
+  int g_nbr = 0;
+  // side streams and fork/join events of the branched capture
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> events;
 };
 
 static void plan_free(mcint_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (hipStream_t st : p->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : p->events) (void)hipEventDestroy(ev);
   dfree(p->tasks_d);
   dfree(p->task_id_d);
   dfree(p->pair_off_d);
@@ -527,6 +560,15 @@ extern "C" int mcint_tune(int key, int value) {
     }
     const int old = g_prims_per_lane;
     g_prims_per_lane = value;
+    return old;
+  }
+  if (key == MCINT_TUNE_GRAPH_BRANCHES) {
+    if (value < 1 || value > MAX_GRAPH_BRANCHES) {
+      set_err("mcint_tune: MCINT_TUNE_GRAPH_BRANCHES takes 1 to 8");
+      return MCINT_ERR_ARG;
+    }
+    const int old = g_graph_branches;
+    g_graph_branches = value;
     return old;
   }
   if (key == MCINT_TUNE_GENERIC_ONLY) {
@@ -705,13 +747,18 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   const KeyIndex kindex(ukeys);
   const size_t nkeys = ukeys.size();
   std::vector<int64_t> tcount((size_t)nthr * nkeys, 0);
+  std::vector<int64_t> pcount((size_t)nthr * nkeys, 0);  // primitive quartets
   parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
     int64_t* c = tcount.data() + (size_t)th * nkeys;
+    int64_t* pc = pcount.data() + (size_t)th * nkeys;
     for (size_t q = q0; q < q1; ++q) {
       if (!qnt[q]) continue;
       const uint32_t id = kindex.find(qkey[q]);
       qkey[q] = id;
       c[id] += qnt[q];
+      int64_t np = qnt[q];
+      for (int a = 0; a < naxes; ++a) np *= bas[(size_t)shls[4 * q + a] * BAS_SLOTS + NPRIM_OF];
+      pc[id] += np;
     }
   });
   // bucket starts, and each thread's cursor inside every bucket
@@ -806,6 +853,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     bk.fp32 = (k >> 26) & 1;
     bk.t0 = cnt[i];
     bk.t1 = cnt[i + 1];
+    int64_t np = 0;
+    for (int th = 0; th < nthr; ++th) np += pcount[(size_t)th * nkeys + i];
+    bk.cost = bucket_cost(kind, bk.mode, bk.la, bk.lb, bk.lc, bk.ld, np, bk.t1 - bk.t0);
     if (bk.fp32) p->stats_fp32 += bk.t1 - bk.t0;
     bk.sstride = 0;
     if (bk.mode == 2) {
@@ -974,14 +1024,37 @@ struct ScratchView {
   int64_t nflags = 0;
 };
 
-static ScratchView plan_scratch(const mcint_plan* p) {
-  return ScratchView{p->scratch_d, p->scratch_size, p->flags_d, p->flags_size};
+// region k of the plan's scratch (one per concurrent graph branch)
+static ScratchView plan_scratch(const mcint_plan* p, int k = 0) {
+  return ScratchView{p->scratch_d ? p->scratch_d + (size_t)k * p->scratch_size : nullptr,
+                     p->scratch_size, p->flags_d ? p->flags_d + (size_t)k * p->flags_size : nullptr,
+                     p->flags_size};
 }
 
-// allocates the plan's scratch on first use
-static int ensure_scratch(mcint_plan* p) {
-  if (p->scratch_size > 0 && !p->scratch_d) HIPCHK(dalloc(&p->scratch_d, (size_t)p->scratch_size));
-  if (p->flags_size > 0 && !p->flags_d) HIPCHK(dalloc(&p->flags_d, (size_t)p->flags_size));
+// allocates the plan's scratch on first use, nbr regions; growing it drops
+// the captured graph (which holds the old addresses)
+static int ensure_scratch(mcint_plan* p, int nbr = 1) {
+  if (p->scratch_nbr >= nbr) return MCINT_OK;
+  if (p->scratch_size == 0 && p->flags_size == 0) {
+    p->scratch_nbr = nbr;
+    return MCINT_OK;
+  }
+  if (p->gexec) {
+    HIPCHK(hipDeviceSynchronize());
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  if (p->scratch_d) HIPCHK(hipDeviceSynchronize());
+  dfree(p->scratch_d);
+  dfree(p->flags_d);
+  p->scratch_nbr = 0;
+  if (p->scratch_size > 0) HIPCHK(dalloc(&p->scratch_d, (size_t)p->scratch_size * nbr));
+  if (p->flags_size > 0) HIPCHK(dalloc(&p->flags_d, (size_t)p->flags_size * nbr));
+  p->scratch_nbr = nbr;
   return MCINT_OK;
 }
 
@@ -1028,6 +1101,58 @@ static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t strea
     const int st = bucket_launch(p, bk, out, not0, stream, S);
     if (st) return st;
   }
+  return MCINT_OK;
+}
+
+// Every bucket of p forked over nbr side streams that join back into
+// 'stream' (captured, the graph holds nbr concurrent branches).  Buckets go
+// to the least loaded branch in descending estimated cost; each branch runs
+// its buckets one after another with its own scratch region.  The joins are
+// recorded even when a launch fails, so that a capture can always end.
+static int plan_launch_branched(mcint_plan* p, double* out, FINT* not0, hipStream_t stream,
+                                int nbr) {
+  while ((int)p->side.size() < nbr) {
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    p->side.push_back(st);
+  }
+  while ((int)p->events.size() < nbr + 1) {
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    p->events.push_back(ev);
+  }
+  std::vector<int> order(p->buckets.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return p->buckets[a].cost > p->buckets[b].cost; });
+  std::vector<double> load(nbr, 0.0);
+  std::vector<std::vector<int>> lists(nbr);
+  for (int i : order) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lists[k].push_back(i);
+    load[k] += p->buckets[i].cost;
+  }
+  HIPCHK(hipEventRecord(p->events[0], stream));
+  int st = MCINT_OK;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < nbr; ++k) {
+    e = hipStreamWaitEvent(p->side[k], p->events[0], 0);
+    if (e != hipSuccess) break;
+  }
+  if (e == hipSuccess) {
+    for (int k = 0; k < nbr && st == MCINT_OK; ++k)
+      for (int i : lists[k]) {
+        st = bucket_launch(p, p->buckets[i], out, not0, p->side[k], plan_scratch(p, k));
+        if (st) break;
+      }
+  }
+  for (int k = 0; k < nbr; ++k) {
+    const hipError_t a = hipEventRecord(p->events[k + 1], p->side[k]);
+    const hipError_t b = hipStreamWaitEvent(stream, p->events[k + 1], 0);
+    if (e == hipSuccess) e = a != hipSuccess ? a : b;
+  }
+  if (st != MCINT_OK) return st;
+  HIPCHK(e);
   return MCINT_OK;
 }
 
@@ -1333,15 +1458,21 @@ extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev
     return MCINT_ERR_ARG;
   }
   hipStream_t stream = (hipStream_t)hip_stream;
-  if (const int st = ensure_scratch(p)) return st;
-  if (!stream) return plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
-  if (!p->gexec || p->g_out != out_dev || p->g_not0 != not0_dev || p->g_stream != stream) {
+  if (!stream) {
+    if (const int st = ensure_scratch(p)) return st;
+    return plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
+  }
+  const int nbr = std::max(1, std::min<int>(g_graph_branches, (int)p->buckets.size()));
+  if (const int st = ensure_scratch(p, nbr)) return st;
+  if (!p->gexec || p->g_out != out_dev || p->g_not0 != not0_dev || p->g_stream != stream ||
+      p->g_nbr != nbr) {
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->gexec = nullptr;
     p->graph = nullptr;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    int st = plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
+    int st = nbr > 1 ? plan_launch_branched(p, out_dev, not0_dev, stream, nbr)
+                     : plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (st != MCINT_OK) {
@@ -1354,6 +1485,7 @@ extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev
     p->g_out = out_dev;
     p->g_not0 = not0_dev;
     p->g_stream = stream;
+    p->g_nbr = nbr;
   }
   HIPCHK(hipGraphLaunch(p->gexec, stream));
   return MCINT_OK;

@@ -361,7 +361,8 @@ def test_screened_quartet_is_zero_filled(mc, orc):
 
 def test_plan_graph_replay_matches_direct(mc):
     """Plan outputs via HIP-graph replays (with eager work and syncs between)
-    equal the directly launched plan, on every replay."""
+    equal the directly launched plan, on every replay, with the buckets in
+    one graph branch and forked over several concurrent branches."""
     import torch
     from libcint_amd.synthetic import make_molecule, unique_quartets
     m = make_molecule("c0")
@@ -375,14 +376,21 @@ def test_plan_graph_replay_matches_direct(mc):
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
     out = torch.empty(plan.out_size, dtype=torch.float64, device=dev)
-    for rep in range(3):
-        out.fill_(float("nan"))
-        torch.cuda.synchronize()
-        plan.execute(out.data_ptr(), stream_ptr=s.cuda_stream, graph=True)
-        s.synchronize()
-        junk = torch.randn(1000, device=dev).sum()  # eager work between replays
-        torch.cuda.synchronize()
-        assert torch.equal(out, ref), f"replay {rep}"
+    # one branch, then the buckets forked over 4 and 3 concurrent branches
+    # (each capture replayed three times)
+    for nbr in (1, 4, 3):
+        old = mc.tune(mc.TUNE_GRAPH_BRANCHES, nbr)
+        try:
+            for rep in range(3):
+                out.fill_(float("nan"))
+                torch.cuda.synchronize()
+                plan.execute(out.data_ptr(), stream_ptr=s.cuda_stream, graph=True)
+                s.synchronize()
+                junk = torch.randn(1000, device=dev).sum()  # eager work between replays
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), f"{nbr} branches, replay {rep}"
+        finally:
+            mc.tune(mc.TUNE_GRAPH_BRANCHES, old)
     del junk
 
 
