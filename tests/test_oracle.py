@@ -428,3 +428,33 @@ def test_int3c2e_derivative_checksums(orc, fixture_basis, name, ref):
     out, _, _ = orc.batch(name, q, orc.Basis(*fixture_basis))
     v = float(np.abs(out).sum())
     assert close_place(v, ref, out.size, 10), (name, v, ref)
+
+
+def test_oracle_under_address_and_undefined_behaviour_sanitizers(golden, fixture_basis, tmp_path):
+    """The oracle built with -fsanitize=address,undefined (findings abort)
+    over libcint's test-suite loops for every kind, exact and tabulated
+    roots: a clean exit and libcint's golden checksums
+    (testsuite/test_cint.py:478-481, test_3c2e.py:303-320)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "sanitize"], check=True)
+    atm, bas, env = fixture_basis
+    path = tmp_path / "basis.txt"
+    with open(path, "w") as fh:
+        fh.write(f"{atm.shape[0]} {bas.shape[0]} {env.size}\n")
+        fh.write(" ".join(map(str, atm.ravel())) + "\n" + " ".join(map(str, bas.ravel())) + "\n")
+        fh.write(" ".join(repr(float(x)) for x in env) + "\n")
+    r = subprocess.run([os.path.join(ROOT, "oracle", "_build", "oracle_sanitize"), str(path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    sums = {}
+    for line in r.stdout.splitlines():
+        mode, name, v = line.split()
+        sums[(mode, name)] = float(v)
+    gold = {"int2e_sph": (golden["golden"]["cint2e_sph"][0], 7),
+            "int2e_ip1_sph": (golden["golden"]["cint2e_ip1_sph"][0], 7),
+            "int3c2e_sph": (golden["golden"]["cint3c2e_sph"][0], 9),
+            "int3c2e_ip1_sph": (2242.052249221302, 9), "int3c2e_ip2_sph": (1970.982483824248, 9),
+            "int2c2e_sph": (782.3104849606677, 9)}
+    for mode in ("exact", "table"):
+        for name, (ref, place) in gold.items():
+            assert round(abs(sums[(mode, name)] - ref), place - 2) == 0, (mode, name, sums[(mode, name)], ref)
