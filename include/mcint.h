@@ -74,6 +74,8 @@ typedef struct {
 #define MCINT_INT3C2E_IP2_CART 9
 #define MCINT_INT2C2E_SPH 10
 #define MCINT_INT2C2E_CART 11
+#define MCINT_INT2E_IP2_SPH 12
+#define MCINT_INT2E_IP2_CART 13
 
 /* ---- 1. libcint-named per-call entry points --------------------------
  * Signature of libcint's CINTIntegralFunction (include/cint_funcs.h:10-17).
@@ -99,6 +101,13 @@ CACHE_SIZE_T int3c2e_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT na
 CACHE_SIZE_T int3c2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
                           FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
                           /* replaces src/cint3c2e.c:709-717 */
+/* (i j|d/dr k l), 3 components, derivative on the third shell: out
+ * (di,dj,dk,dl,3); replaces src/autocode/grad2.c:155-170 (cart 155-162,
+ * sph 163-170).  Runs on the i-derivative kernels as (d/dr k l|i j). */
+CACHE_SIZE_T int2e_ip2_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                           FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int2e_ip2_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                            FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
 /* (d/dr i j|k), 3 components, derivative on the first shell: out
  * (di,dj,dk,3); replaces src/autocode/int3c2e.c:72-87 (cart 72-79, sph 80-87). */
 CACHE_SIZE_T int3c2e_ip1_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
@@ -122,6 +131,8 @@ CACHE_SIZE_T int2c2e_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT n
  * src/cint3c2e.c:703-707) and their destructors (src/optimizer.c:40-72). */
 void int2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void int2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+/* src/autocode/grad2.c:151-154 */
+void int2e_ip2_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void int3c2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 /* src/autocode/int3c2e.c:68-71, 149-152; src/cint2c2e.c:360-366 */
 void int3c2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
@@ -217,6 +228,13 @@ int int2c2e_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, 
 int int2c2e_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
                        const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
                        const CINTOpt *opt, FINT *not0, void *hip_stream);
+/* int2e_ip2 over a list of quartets (blocks (di,dj,dk,dl,3)) */
+int int2e_ip2_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                        const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                        const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_ip2_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                         const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
+                         const CINTOpt *opt, FINT *not0, void *hip_stream);
 
 /* ---- 3. device-resident plans ------------------------------------------- */
 typedef struct mcint_basis mcint_basis;

@@ -21,10 +21,11 @@ LIB_PATH = os.path.join(HERE, "libmcint.so")
 
 INTOR_NAMES = ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
                "int3c2e_cart", "int3c2e_ip1_sph", "int3c2e_ip1_cart", "int3c2e_ip2_sph",
-               "int3c2e_ip2_cart", "int2c2e_sph", "int2c2e_cart")
+               "int3c2e_ip2_cart", "int2c2e_sph", "int2c2e_cart", "int2e_ip2_sph", "int2e_ip2_cart")
 INTOR_CODES = {n: i for i, n in enumerate(INTOR_NAMES)}
 (INT2E_SPH, INT2E_CART, INT2E_IP1_SPH, INT2E_IP1_CART, INT3C2E_SPH, INT3C2E_CART, INT3C2E_IP1_SPH,
- INT3C2E_IP1_CART, INT3C2E_IP2_SPH, INT3C2E_IP2_CART, INT2C2E_SPH, INT2C2E_CART) = range(12)
+ INT3C2E_IP1_CART, INT3C2E_IP2_SPH, INT3C2E_IP2_CART, INT2C2E_SPH, INT2C2E_CART, INT2E_IP2_SPH,
+ INT2E_IP2_CART) = range(14)
 
 
 def intor_shape(name: str):
@@ -121,7 +122,8 @@ def lib():
     L.mcint_tune_class.argtypes = [ctypes.c_int] * 6
     L.mcint_last_error.restype = ctypes.c_char_p
     for name in ("int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer",
-                 "int3c2e_ip1_optimizer", "int3c2e_ip2_optimizer", "int2c2e_optimizer"):
+                 "int3c2e_ip1_optimizer", "int3c2e_ip2_optimizer", "int2c2e_optimizer",
+                 "int2e_ip2_optimizer"):
         f = getattr(L, name)
         f.restype = None
         f.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_int, ip, ctypes.c_int, dp]

@@ -66,7 +66,9 @@ int intor_kind(int intor) {
     case MCINT_INT2E_SPH:
     case MCINT_INT2E_CART: return KIND_2E;
     case MCINT_INT2E_IP1_SPH:
-    case MCINT_INT2E_IP1_CART: return KIND_2E_IP1;
+    case MCINT_INT2E_IP1_CART:
+    case MCINT_INT2E_IP2_SPH:  // the ip1 kernels with the ket pair first
+    case MCINT_INT2E_IP2_CART: return KIND_2E_IP1;
     case MCINT_INT3C2E_SPH:
     case MCINT_INT3C2E_CART: return KIND_3C2E;
     case MCINT_INT3C2E_IP1_SPH:
@@ -81,6 +83,10 @@ int intor_kind(int intor) {
 
 // spherical output: the even intor codes
 int intor_sph(int intor) { return intor >= 0 && intor % 2 == 0; }
+
+// int2e_ip2, (ij|d/dr k l) = (d/dr k l|ij): derivative on the ket's first
+// shell, computed by the i-derivative kernels on the axes (k, l, i, j)
+int intor_ket_der(int intor) { return intor == MCINT_INT2E_IP2_SPH || intor == MCINT_INT2E_IP2_CART; }
 
 }  // namespace
 
@@ -276,7 +282,8 @@ struct QuartetShape {
 // Shell data of one shell tuple s (naxes = 3 for int3c2e) and its
 // canonical axis order: 0, or 1 when a shell index is out of range, 2 when
 // the class needs more Rys roots than the device tables hold.
-int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, QuartetShape& Q) {
+int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, QuartetShape& Q,
+                   int ket_der = 0) {
   const int naxes = kind_axes(kind);
   for (int a = 0; a < 4; ++a) {
     Q.l[a] = 0;
@@ -305,6 +312,13 @@ int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, Q
       std::swap(L[0], L[2]);
       std::swap(L[1], L[3]);
     }
+  } else if (kind == KIND_2E_IP1 && ket_der) {
+    // derivative shell k first, then l; the bra pair ordered by l
+    L[0] = 2;
+    L[1] = 3;
+    L[2] = 0;
+    L[3] = 1;
+    if (l[0] < l[1]) std::swap(L[2], L[3]);
   } else if (kind == KIND_2E_IP1) {
     if (l[2] < l[3]) std::swap(L[2], L[3]);
   } else if (kind == KIND_3C2E || kind == KIND_3C2E_IP2) {
@@ -642,7 +656,7 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   std::vector<int64_t> nprim_thr(nthr, 0), ntask_thr(nthr, 0);
   const bool reg_ok = (b->omega >= 0 || kind != KIND_2E_IP1) && !g_generic_only;
   auto describe = [&](size_t q, QuartetShape& Q) -> int {
-    return describe_tuple(bas, nbas, kind, sph, shls + 4 * q, Q);
+    return describe_tuple(bas, nbas, kind, sph, shls + 4 * q, Q, intor_ket_der(intor));
   };
   parallel_for(nthr, nq, [&](int th, size_t q0, size_t q1) {
     // per-thread totals kept in registers (the shared arrays are written
@@ -1629,6 +1643,8 @@ MC_BATCH(int3c2e_ip2_sph_batch, MCINT_INT3C2E_IP2_SPH)
 MC_BATCH(int3c2e_ip2_cart_batch, MCINT_INT3C2E_IP2_CART)
 MC_BATCH(int2c2e_sph_batch, MCINT_INT2C2E_SPH)
 MC_BATCH(int2c2e_cart_batch, MCINT_INT2C2E_CART)
+MC_BATCH(int2e_ip2_sph_batch, MCINT_INT2E_IP2_SPH)
+MC_BATCH(int2e_ip2_cart_batch, MCINT_INT2E_IP2_CART)
 #undef MC_BATCH
 
 // ---------------------------------------------------------------------------
@@ -1832,7 +1848,7 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
   const int naxes = kind_axes(kind);
   const int ncomp = kind_ncomp(kind);
   QuartetShape Q;
-  const int why = describe_tuple(b->bas.data(), b->nbas, kind, sph, shls, Q);
+  const int why = describe_tuple(b->bas.data(), b->nbas, kind, sph, shls, Q, intor_ket_der(intor));
   if (why) {
     set_err(why == 1 ? "shell index out of range" : "more Rys roots than the device tables hold");
     return -1;
@@ -2019,6 +2035,8 @@ MC_INTOR(int3c2e_ip2_sph, MCINT_INT3C2E_IP2_SPH)
 MC_INTOR(int3c2e_ip2_cart, MCINT_INT3C2E_IP2_CART)
 MC_INTOR(int2c2e_sph, MCINT_INT2C2E_SPH)
 MC_INTOR(int2c2e_cart, MCINT_INT2C2E_CART)
+MC_INTOR(int2e_ip2_sph, MCINT_INT2E_IP2_SPH)
+MC_INTOR(int2e_ip2_cart, MCINT_INT2E_IP2_CART)
 #undef MC_INTOR
 
 extern "C" void int2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
@@ -2026,6 +2044,10 @@ extern "C" void int2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, 
   make_opt(opt, KIND_2E, atm, natm, bas, nbas, env);
 }
 extern "C" void int2e_ip1_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                    double* env) {
+  make_opt(opt, KIND_2E_IP1, atm, natm, bas, nbas, env);
+}
+extern "C" void int2e_ip2_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
                                     double* env) {
   make_opt(opt, KIND_2E_IP1, atm, natm, bas, nbas, env);
 }

@@ -38,7 +38,8 @@ def lib():
         for name in ("orc_int2e_sph", "orc_int2e_cart", "orc_int2e_ip1_sph",
                      "orc_int2e_ip1_cart", "orc_int3c2e_sph", "orc_int3c2e_cart",
                      "orc_int3c2e_ip1_sph", "orc_int3c2e_ip1_cart", "orc_int3c2e_ip2_sph",
-                     "orc_int3c2e_ip2_cart", "orc_int2c2e_sph", "orc_int2c2e_cart"):
+                     "orc_int3c2e_ip2_cart", "orc_int2c2e_sph", "orc_int2c2e_cart",
+                     "orc_int2e_ip2_sph", "orc_int2e_ip2_cart"):
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp]
@@ -120,7 +121,8 @@ class Basis:
 
 _KIND = {"int2e_sph": 0, "int2e_cart": 1, "int2e_ip1_sph": 2, "int2e_ip1_cart": 3,
          "int3c2e_sph": 4, "int3c2e_cart": 5, "int3c2e_ip1_sph": 6, "int3c2e_ip1_cart": 7,
-         "int3c2e_ip2_sph": 8, "int3c2e_ip2_cart": 9, "int2c2e_sph": 10, "int2c2e_cart": 11}
+         "int3c2e_ip2_sph": 8, "int3c2e_ip2_cart": 9, "int2c2e_sph": 10, "int2c2e_cart": 11,
+         "int2e_ip2_sph": 12, "int2e_ip2_cart": 13}
 
 
 def _shape(name):

@@ -457,7 +457,7 @@ typedef struct {
         int three_center;
         int two_center; /* 2c2e: the j slot is an exponent-0 s function too */
         int deriv_i; /* 1 for int2e_ip1 / int3c2e_ip1 */
-        int deriv_k; /* 1 for int3c2e_ip2 (derivative on the auxiliary shell) */
+        int deriv_k; /* 1 for int3c2e_ip2 (the auxiliary shell) or int2e_ip2 (shell k) */
         int ncomp;
 } OcEnv;
 
@@ -503,8 +503,11 @@ static void oc_strides(OcEnv *e)
         else e->rxkl = e->kbase ? e->r[2] : e->r[3];
 }
 
+/* Four-centre (ij|kl), src/g2e.c:21-172; deriv_i raises i for int2e_ip1
+ * (ng = {1,0,0,0,...}, src/autocode/grad2.c:69-88), deriv_k raises k for
+ * int2e_ip2 (ng = {0,0,1,0,...}, src/autocode/grad2.c:151-170). */
 static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas,
-                       const double *env, int deriv_i)
+                       const double *env, int deriv_i, int deriv_k)
 {
         memset(e, 0, sizeof(*e));
         for (int s = 0; s < 4; s++) {
@@ -514,8 +517,10 @@ static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas
                 e->r[s] = env + ATM(PTR_COORD, BAS(ATOM_OF, shls[s]));
         }
         e->deriv_i = deriv_i;
+        e->deriv_k = deriv_k;
         e->lc[0] += deriv_i;
-        e->ncomp = deriv_i ? 3 : 1;
+        e->lc[2] += deriv_k;
+        e->ncomp = (deriv_i || deriv_k) ? 3 : 1;
         e->nfall = e->nf[0] * e->nf[1] * e->nf[2] * e->nf[3];
         e->common_factor = (OC_PI * OC_PI * OC_PI) * 2 / OC_SQRTPI *
                            common_fac_sp(e->l[0]) * common_fac_sp(e->l[1]) *
@@ -807,6 +812,28 @@ static void oc_nabla_i(double *f, const double *g, const OcEnv *e, double ai)
         }
 }
 
+/* nabla on k of a four-centre g: f(k) = k g(k-1) - 2 ak g(k+1)
+ * (CINTnabla1k_2e, src/g2e.c:4651-4697; G2E_D_K of src/autocode/grad2.c:107) */
+static void oc_nabla_k(double *f, const double *g, const OcEnv *e, double ak)
+{
+        int nr = e->nroots;
+        double ak2 = -2 * ak;
+        for (int d = 0; d < 3; d++) {
+                const double *gd = g + d * e->gsize;
+                double *fd = f + d * e->gsize;
+                for (int j = 0; j <= e->l[1]; j++)
+                for (int l = 0; l <= e->l[3]; l++)
+                for (int k = 0; k <= e->l[2]; k++)
+                for (int i = 0; i <= e->l[0]; i++)
+                for (int r = 0; r < nr; r++) {
+                        int p = r + i * e->gsi + k * e->gsk + l * e->gsl + j * e->gsj;
+                        double v = ak2 * gd[p + e->gsk];
+                        if (k > 0) v += k * gd[p - e->gsk];
+                        fd[p] = v;
+                }
+        }
+}
+
 static void oc_gout_ip1(double *gout, const double *g, const double *f, const int *idx,
                         const OcEnv *e)
 {
@@ -937,7 +964,8 @@ static void oc_c2s_4(double *out, const double *gblk, const int *lv, const int *
 /* src/cint2e.c:57-269 (loop), 794-870 (drv)                           */
 /* ------------------------------------------------------------------ */
 /* three_center: 1 for (ij|k); 2 for the two-centre (i|k) of int2c2e, whose
- * shls are (i, k).  deriv_k: derivative on the auxiliary shell (3c only). */
+ * shls are (i, k).  deriv_k: derivative on the auxiliary shell (3c) or on k
+ * (four centres, int2e_ip2). */
 static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const int *atm,
                         const int *bas, const double *env, int deriv_i, int sph, int three_center,
                         int deriv_k)
@@ -949,7 +977,7 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const 
         else if (!three_center) shls[3] = shls_in[3];
         int sh[4] = {shls[0], shls[1], shls[2], three_center ? shls[2] : shls[3]};
         if (three_center) oc_init_3c2e(&e, shls, atm, bas, env, deriv_i, deriv_k, two_center);
-        else oc_init_2e(&e, shls, atm, bas, env, deriv_i);
+        else oc_init_2e(&e, shls, atm, bas, env, deriv_i, deriv_k);
 
         int nprim[4], nctr[4];
         const double *ex[4], *cf[4];
@@ -1067,7 +1095,8 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const 
                                         oc_nabla_i(f, g, &e, ex[0][ip]);
                                         oc_gout_ip1(gout, g, f, idx, &e);
                                 } else if (deriv_k) {
-                                        oc_nabla_aux(f, g, &e, ex[2][kp]);
+                                        if (three_center) oc_nabla_aux(f, g, &e, ex[2][kp]);
+                                        else oc_nabla_k(f, g, &e, ex[2][kp]);
                                         oc_gout_ip1(gout, g, f, idx, &e);
                                 } else {
                                         oc_gout_plain(gout, g, idx, &e);
@@ -1156,6 +1185,19 @@ int orc_int2e_ip1_cart(double *out, const int *dims, const int *shls, const int 
         (void)natm; (void)nbas;
         return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 0, 0, 0);
 }
+/* (ij|d/dr k l), src/autocode/grad2.c:101-170 */
+int orc_int2e_ip2_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                      const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 1, 0, 1);
+}
+int orc_int2e_ip2_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                       const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 0, 1);
+}
 int orc_int3c2e_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
                     const int *bas, int nbas, const double *env)
 {
@@ -1211,7 +1253,7 @@ int orc_int2c2e_cart(double *out, const int *dims, const int *shls, const int *a
 /* Batch driver shaped like examples/time_c2h6.c:719-745: OpenMP over a list
  * of shell quartets, dynamic schedule.  kind: 0 sph, 1 cart, 2 ip1_sph,
  * 3 ip1_cart, 4 3c2e_sph, 5 3c2e_cart, 6/7 3c2e_ip1 sph/cart, 8/9 3c2e_ip2
- * sph/cart, 10/11 2c2e sph/cart (shls stride 4; 4th ignored for 3c, the
+ * sph/cart, 10/11 2c2e sph/cart, 12/13 int2e_ip2 sph/cart (shls stride 4; 4th ignored for 3c, the
  * first two used for 2c).  out[off[q]] receives
  * quartet q packed.  Returns the number of quartets whose status was < 0. */
 int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_t nq,
@@ -1238,6 +1280,8 @@ int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_
                 case 8: st = orc_int3c2e_ip2_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 9: st = orc_int3c2e_ip2_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 10: st = orc_int2c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 12: st = orc_int2e_ip2_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 13: st = orc_int2e_ip2_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 default: st = orc_int2c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 }
                 if (not0) not0[q] = st;

@@ -58,7 +58,8 @@ def test_device_rys_roots(mc, orc):
 
 @pytest.mark.parametrize("name,key", [("int2e_sph", "cint2e_sph"), ("int2e_cart", None),
                                       ("int2e_ip1_sph", "cint2e_ip1_sph"),
-                                      ("int2e_ip1_cart", None)])
+                                      ("int2e_ip1_cart", None), ("int2e_ip2_sph", None),
+                                      ("int2e_ip2_cart", None)])
 def test_libcint_fixture(mc, orc, golden, fixture_basis, name, key):
     """All (i<=j, k<=l) quartets of testsuite/test_cint.py's fixture: l <= 4,
     a 2-primitive 2-contraction d shell, up to 9 Rys roots."""
@@ -227,9 +228,11 @@ def test_synthetic_samples(mc, orc, cfg, n):
     _say(cfg, "sample of", n, "quartets")
     orc.set_root_mode(orc.ROOTS_TABLE)
     try:
-        for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart"):
+        for name in ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart",
+                     "int2e_ip2_sph", "int2e_ip2_cart"):
             qq = {"int2e_sph": q, "int2e_cart": q[: n // 2], "int2e_ip1_sph": q[: n // 4],
-                  "int2e_ip1_cart": q[: n // 8]}[name]
+                  "int2e_ip1_cart": q[: n // 8], "int2e_ip2_sph": q[n // 4: n // 2],
+                  "int2e_ip2_cart": q[n // 2: n // 2 + n // 8]}[name]
             got, _, nz = mc.batch(name, qq, mc.Basis(m.atm, m.bas, m.env))
             ref, _, nz_r = orc.batch(name, qq, orc.Basis(m.atm, m.bas, m.env))
             _cmp(got, ref, f"{cfg} {name}")
@@ -299,7 +302,7 @@ def test_per_call_threads_share_optimizer(mc, fixture_basis):
     dp = ctypes.POINTER(ctypes.c_double)
     args = (B.atm.ctypes.data_as(ip), B.natm, B.bas.ctypes.data_as(ip), B.nbas,
             B.env.ctypes.data_as(dp))
-    for name in ("int2e_sph", "int2e_ip1_sph"):
+    for name in ("int2e_sph", "int2e_ip1_sph", "int2e_ip2_sph"):
         q = libcint_quartets()
         ref, off, nz_ref = mc.batch(name, q, B)
         opt = ctypes.c_void_p()
