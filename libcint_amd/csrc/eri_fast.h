@@ -21,6 +21,8 @@ namespace ct {
 #undef C2S_QUAL
 }  // namespace ct
 
+#include "chunk_order.inc"
+
 // ---- compile-time component bookkeeping --------------------------------
 __host__ __device__ constexpr int csum(int lo, int hi) {
   int s = 0;
@@ -168,18 +170,24 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Row starts of the screening staircase: STAIR_ROWS_PER_LANE * T + 1 ints
-// per team of T lanes (teams with more rows test every quartet instead).
+// per team of T lanes (teams with more rows test every quartet instead);
+// a workgroup of n threads needs at most (STAIR_ROWS_PER_LANE + 1) * n.
 constexpr int STAIR_ROWS_PER_LANE = 8;
 constexpr int STAIR_LDS_INTS = STAIR_ROWS_PER_LANE * 256 + 256;
+constexpr int stair_lds_ints(int threads) { return (STAIR_ROWS_PER_LANE + 1) * threads; }
 __device__ __forceinline__ int* stair_rows(int* s, int T) {
   return s + (threadIdx.x / T) * (STAIR_ROWS_PER_LANE * T + 1);
 }
 
-// Primitive loop of one team: accumulates the [e0|f0] components with flat
-// index (i*NF + j) in [K0, K1) over this lane's primitive quartets.  'live'
-// is the team's pair-emptiness verdict, 'any' whether this lane saw a
-// primitive quartet that passed libcint's two screening tests.
-template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0>
+// Primitive loop of one team: accumulates the [e0|f0] components at
+// positions [K0, K1) of the component order over this lane's primitive
+// quartets; the order is the flat index (i*NF + j) when ORDW = 0, else
+// ChunkOrder<LA,LB,LC,LD,ORDW> (chunks of ORDW components grouped to share
+// recurrence entries).  'live' is the team's pair-emptiness verdict, 'any'
+// whether this lane saw a primitive quartet that passed libcint's two
+// screening tests.
+template <int LA, int LB, int LC, int LD, int K0, int K1, typename R = double, int DER = 0,
+          int ORDW = 0>
 __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, int T, int tl,
                                            const double* __restrict__ rtab, int* __restrict__ rows,
                                            R (&acc)[K1 - K0], int& any, int& live) {
@@ -381,7 +389,8 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
                      R(b00));
       sfor<K1 - K0>([&](auto KK) {
         constexpr int k = K0 + decltype(KK)::value;
-        constexpr int i = k / NF, j = k % NF;
+        constexpr int kf = ORDW ? ChunkOrder<LA, LB, LC, LD, ORDW>::at(k) : k;
+        constexpr int i = kf / NF, j = kf % NF;
         constexpr int lo = C::row_lo(i), ik = C::row_k(i);
         constexpr int ex = cpx(lo, ik), ey = cpy(lo, ik), ez = cpz(lo, ik);
         constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
@@ -629,33 +638,48 @@ constexpr int chunk_target(int la, int lb, int lc, int ld, int der) {
 template <int LA, int LB, int LC, int LD, int DER = 0>
 struct ChunkCfg {
   using C = FastClass<LA, LB, LC, LD, DER>;
+  static constexpr int THREADS = 256;
   static constexpr int NEF = C::NE * C::NF;
   static constexpr int TGT = chunk_target(LA, LB, LC, LD, DER);
   static constexpr int NCH = (NEF + TGT - 1) / TGT;
   static constexpr int W = (NEF + NCH - 1) / NCH;
   static constexpr int k0(int c) { return c * W; }
   static constexpr int k1(int c) { return (c + 1) * W < NEF ? (c + 1) * W : NEF; }
+  // component order of the chunks (flat for the derivative classes, whose
+  // rows are the raised and lowered sets, and for classes whose epilogue
+  // reads the block from global memory, where gathering through the map
+  // costs more than the order saves).  The per-task scratch block holds
+  // the components in this order, each chunk's range contiguous; pos() maps
+  // a flat index to its position (nullptr: the identity).
+  static constexpr bool EPI_GLOBAL =
+      NEF + nout(LA, true) * nout(LB, true) * C::NF > 6144;
+  static constexpr int ORDW =
+      DER != 1 && NCH > 1 && !EPI_GLOBAL && ChunkOrder<LA, LB, LC, LD, W>::HAS ? W : 0;
+  __device__ static const short* pos() {
+    if constexpr (ORDW != 0) return ChunkOrder<LA, LB, LC, LD, ORDW>::pos();
+    return nullptr;
+  }
 };
 
 template <int LA, int LB, int LC, int LD, int CH, int DER = 0, typename R = double>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((ChunkCfg<LA, LB, LC, LD, DER>::THREADS))
 __attribute__((amdgpu_waves_per_eu(min_waves(LA, LB, LC, LD, DER, CH))))
 eri_chunk_kernel(BasisView Bv, BucketArgs A) {
   using G = ChunkCfg<LA, LB, LC, LD, DER>;
   constexpr int K0 = G::k0(CH), K1 = G::k1(CH);
   __shared__ double s_rys[RysLds<G::C::NR>::N];
-  __shared__ int s_rows[DER == 2 || DER == 3 ? 1 : STAIR_LDS_INTS];
+  __shared__ int s_rows[DER == 2 || DER == 3 ? 1 : stair_lds_ints(G::THREADS)];
   const double* rtab = rys_stage_table<G::C::NR>(s_rys);
   const int T = A.team;
   const int tid = threadIdx.x;
   const int tl = tid & (T - 1);
-  const int64_t t = A.t0 + (int64_t)blockIdx.x * (256 / T) + tid / T;
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * (G::THREADS / T) + tid / T;
   if (t >= A.t1) return;
   const Task tk = A.tasks[t];
   R acc[K1 - K0];
   int any, live;
-  accumulate<LA, LB, LC, LD, K0, K1, R, DER>(Bv, tk, T, tl, rtab, stair_rows(s_rows, T), acc, any,
-                                             live);
+  accumulate<LA, LB, LC, LD, K0, K1, R, DER, G::ORDW>(Bv, tk, T, tl, rtab, stair_rows(s_rows, T), acc,
+                                                      any, live);
   for (int off = 1; off < T; off <<= 1) {
     sfor<K1 - K0>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -746,7 +770,7 @@ struct EpiCfg {
 template <int LA, int LB, int LC, int LD, bool SPH = true>
 __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
                                         const double* __restrict__ X, double* __restrict__ Ys,
-                                        int j0, int js) {
+                                        int j0, int js, const short* __restrict__ pos = nullptr) {
   using C = FastClass<LA, LB, LC, LD>;
   constexpr int NF = C::NF, NA = C::NA, NB = C::NB, NAo = nout(LA, SPH), NBo = nout(LB, SPH);
   const VShell& sa = Bv.vsh[tk.sh[0]];
@@ -771,7 +795,8 @@ __device__ __forceinline__ void epi_bra(const BasisView& Bv, const Task& tk,
         constexpr double cb = binomc(bx, tx) * binomc(by, ty) * binomc(bz, tz);
         constexpr int e = cindex(LA, ax + tx, ay + ty, az + tz);
         const double pw = abp[0][bx - tx] * abp[1][by - ty] * abp[2][bz - tz];
-        s = fma(cb * pw, X[e * NF + j], s);
+        const int xi = pos ? pos[e * NF + j] : e * NF + j;
+        s = fma(cb * pw, X[xi], s);
       });
       Yc[a * NB + b] = s;
     });
@@ -851,11 +876,14 @@ __device__ __forceinline__ void epi_ket_store(const BasisView& Bv, const Task& t
   }
 }
 
-template <int LA, int LB, int LC, int LD, bool SPH = true>
+// The per-task block is read in the component order of the chunks that
+// wrote it (ChunkCfg<..., DER>::pos(): the position of each flat index).
+template <int LA, int LB, int LC, int LD, bool SPH = true, int DER = 0>
 __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A) {
   using C = FastClass<LA, LB, LC, LD>;
   using P = EpiCfg<LA, LB, LC, LD, SPH>;
   constexpr int NEF = P::NEF, TE = P::TE;
+  const short* __restrict__ pos = ChunkCfg<LA, LB, LC, LD, DER>::pos();
   __shared__ double sm[P::TPB * P::PER];
   const int team = threadIdx.x / TE;
   const int tl = threadIdx.x % TE;
@@ -873,11 +901,11 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
     if constexpr (P::XG) {
       X = src;
     } else if (any) {
-      for (int k = tl; k < NEF; k += TE) sm[team * P::PER + k] = src[k];
+      for (int k = tl; k < NEF; k += TE) sm[team * P::PER + k] = src[pos ? pos[k] : k];
     }
   }
   __syncthreads();
-  if (valid && any) epi_bra<LA, LB, LC, LD, SPH>(Bv, tk, X, Ys, tl, TE);
+  if (valid && any) epi_bra<LA, LB, LC, LD, SPH>(Bv, tk, X, Ys, tl, TE, P::XG ? pos : nullptr);
   __syncthreads();
   if (valid) epi_ket_store<LA, LB, LC, LD, SPH>(Bv, tk, Ys, A.out, any, tl, TE);
 }
@@ -1128,15 +1156,16 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
 
 template <int LA, int LB, int LC, int LD, int CH, int DER>
 hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t stream, dim3 grid) {
+  constexpr int TH = ChunkCfg<LA, LB, LC, LD, DER>::THREADS;
   // the FP32 tier exists for the derivative chunks (gradient digest)
   if constexpr (DER == 1) {
     if (A.fp32) {
-      hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER, float>), grid, dim3(256), 0,
+      hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER, float>), grid, dim3(TH), 0,
                          stream, B, A);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER>), grid, dim3(256), 0, stream, B, A);
+  hipLaunchKernelGGL((eri_chunk_kernel<LA, LB, LC, LD, CH, DER>), grid, dim3(TH), 0, stream, B, A);
   return hipGetLastError();
 }
 
@@ -1144,7 +1173,7 @@ template <int LA, int LB, int LC, int LD, int DER, int... CHs>
 hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t stream,
                          std::integer_sequence<int, CHs...>) {
   const int64_t n = A.t1 - A.t0;
-  const int64_t per = 256 / A.team;
+  const int64_t per = ChunkCfg<LA, LB, LC, LD, DER>::THREADS / A.team;
   const dim3 grid((unsigned)((n + per - 1) / per));
   hipError_t e = hipSuccess;
   ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
@@ -1153,16 +1182,19 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
     if (A.sph) {
       using P = EpiCfg<LA, LB, LC, LD>;
       const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
-      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD, true, DER>), egrid, dim3(P::THREADS), 0,
+                         stream, B, A);
     } else {
       using P = EpiCfg<LA, LB, LC, LD, false>;
       const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
-      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD, false>), egrid, dim3(P::THREADS), 0, stream, B, A);
+      hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD, false, DER>), egrid, dim3(P::THREADS), 0,
+                         stream, B, A);
     }
   } else if constexpr (DER != 1) {
     using P = EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));
-    hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD>), egrid, dim3(P::THREADS), 0, stream, B, A);
+    hipLaunchKernelGGL((eri_epi_kernel<LA, LB, LC, LD, true, DER>), egrid, dim3(P::THREADS), 0,
+                       stream, B, A);
   } else if (A.sph) {
     using P = Ip1EpiCfg<LA, LB, LC, LD>;
     const dim3 egrid((unsigned)((n + P::TPB - 1) / P::TPB));

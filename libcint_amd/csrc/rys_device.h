@@ -62,12 +62,13 @@ __device__ __forceinline__ void rys_root_one(int n, int r, double x, double& u, 
   }
 }
 
-// Root tables small enough to stage in LDS (n <= 6: at most 98 KB).  In
-// LDS each interval's block of RYS_NCOEF * 2n coefficients is padded by two
-// doubles (keeping 16-byte alignment): lanes reading different intervals
-// then start on different LDS banks (an unpadded block is a multiple of 8
-// banks long).
-constexpr int RYS_LDS_NMAX = 6;
+// Root tables staged in LDS (n <= 4: at most 55 KB, which leaves room for
+// two workgroups of 256 threads per CU; the 75-98 KB tables of n = 5, 6
+// would allow only one and are read from the global copy).  In LDS each
+// interval's block of RYS_NCOEF * 2n coefficients is padded by two doubles
+// (keeping 16-byte alignment): lanes reading different intervals then start
+// on different LDS banks (an unpadded block is a multiple of 8 banks long).
+constexpr int RYS_LDS_NMAX = 4;
 template <int NR>
 struct RysLds {
   static constexpr bool IN_LDS = NR <= RYS_LDS_NMAX;
