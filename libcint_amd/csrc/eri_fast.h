@@ -179,6 +179,9 @@ __device__ __forceinline__ int* stair_rows(int* s, int T) {
   return s + (threadIdx.x / T) * (STAIR_ROWS_PER_LANE * T + 1);
 }
 
+#ifndef ROOT_LOOP_DOUBLES
+#define ROOT_LOOP_DOUBLES 96
+#endif
 // Primitive loop of one team: accumulates the [e0|f0] components at
 // positions [K0, K1) of the component order over this lane's primitive
 // quartets; the order is the flat index (i*NF + j) when ORDW = 0, else
@@ -193,6 +196,10 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
                                            R (&acc)[K1 - K0], int& any, int& live) {
   using C = FastClass<LA, LB, LC, LD, DER>;
   constexpr int E = C::E, F = C::F, NR = C::NR, NF = C::NF;
+  // Roots in a loop rather than unrolled once the accumulators and one
+  // root's recurrence tables fill the register file: unrolled, the
+  // compiler overlaps several roots' tables and spills.
+  constexpr bool ROOT_LOOP = (K1 - K0) + 3 * (E + 1) * (F + 1) > ROOT_LOOP_DOUBLES;
   const VShell& sa = Bv.vsh[tk.sh[0]];
   const VShell& sb = Bv.vsh[tk.sh[1]];
   const VShell& sc = Bv.vsh[tk.sh[2]];
@@ -413,7 +420,12 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
         u[r] = ut * rcp_pos(u[r] + 1.0 - ut);
         w[r] *= mst;
       });
-      sfor<2 * NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+      if constexpr (ROOT_LOOP) {
+#pragma unroll 1
+        for (int r = 0; r < 2 * NR; ++r) root_body(u[r], w[r]);
+      } else {
+        sfor<2 * NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+      }
     } else {
       double u[NR], w[NR];
       if (Bv.omega > 0) {
@@ -433,7 +445,12 @@ __device__ __forceinline__ void accumulate(const BasisView& Bv, const Task& tk, 
       } else {
         rys_roots<NR>(a0 * rr, u, w, rtab);
       }
-      sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+      if constexpr (ROOT_LOOP) {
+#pragma unroll 1
+        for (int r = 0; r < NR; ++r) root_body(u[r], w[r]);
+      } else {
+        sfor<NR>([&](auto RI) { root_body(u[decltype(RI)::value], w[decltype(RI)::value]); });
+      }
     }
   }
 }
