@@ -466,9 +466,13 @@ double bucket_cost(int kind, int mode, int la, int lb, int lc, int ld, int64_t n
   return mode == 0 ? 15.0 * c : c;
 }
 
-// Scratch bound of the generic and chunked paths (doubles); buckets larger
-// than the scratch are launched in slices.
-constexpr int64_t GEN_SCRATCH_DOUBLES = 64ll << 20;
+// Scratch bound of the generic and chunked paths (doubles, per concurrent
+// graph branch); buckets larger than the scratch are launched in slices.
+// Slices run one after another, each with fewer waves than the GPU holds,
+// so the bound is sized for whole buckets of large workloads: 4 GiB (the
+// largest c1 bucket, 695,543 (fd|dp) quartets x 736 doubles, fits), 16 GiB
+// over four branches, allocated only up to what the plan needs.
+constexpr int64_t GEN_SCRATCH_DOUBLES = 512ll << 20;
 // copies of the gradient accumulators (spreads the digest's atomics)
 constexpr int GRAD_SLOTS = 64;
 
