@@ -1125,10 +1125,12 @@ static int plan_launch(mcint_plan* p, double* out, FINT* not0, hipStream_t strea
 // Every bucket of p forked over nbr side streams that join back into
 // 'stream' (captured, the graph holds nbr concurrent branches).  Buckets go
 // to the least loaded branch in descending estimated cost; each branch runs
-// its buckets one after another with its own scratch region.  The joins are
-// recorded even when a launch fails, so that a capture can always end.
-static int plan_launch_branched(mcint_plan* p, double* out, FINT* not0, hipStream_t stream,
-                                int nbr) {
+// its buckets one after another with its own scratch region:
+// launch(bucket, branch stream, branch scratch) launches one bucket.  The
+// joins are recorded even when a launch fails, so that a capture can
+// always end.
+template <class F>
+static int launch_branched(mcint_plan* p, hipStream_t stream, int nbr, F&& launch) {
   while ((int)p->side.size() < nbr) {
     hipStream_t st = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1160,7 +1162,7 @@ static int plan_launch_branched(mcint_plan* p, double* out, FINT* not0, hipStrea
   if (e == hipSuccess) {
     for (int k = 0; k < nbr && st == MCINT_OK; ++k)
       for (int i : lists[k]) {
-        st = bucket_launch(p, p->buckets[i], out, not0, p->side[k], plan_scratch(p, k));
+        st = launch(p->buckets[i], p->side[k], plan_scratch(p, k));
         if (st) break;
       }
   }
@@ -1190,33 +1192,46 @@ extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int 
     return MCINT_ERR_ARG;
   }
   hipStream_t stream = (hipStream_t)hip_stream;
-  if (const int st = ensure_scratch(p)) return st;
+  for (const Bucket& bk : p->buckets)
+    if (bk.mode == 1) {
+      set_err("mcint_plan_execute_grad: internal: derivative bucket on the single-kernel path");
+      return MCINT_ERR_UNSUPPORTED;
+    }
+  // the buckets run over concurrent branches (as a captured plan does);
+  // their digests add atomically into the shared accumulators
+  const int nbr = std::max(1, std::min<int>(g_graph_branches, (int)p->buckets.size()));
+  if (const int st = ensure_scratch(p, nbr)) return st;
   const BasisView B = plan_view(p);
   // per-block partial sums go to GRAD_SLOTS copies of the accumulators
   // (block % GRAD_SLOTS), reduced into grad_dev at the end
   const int n = 3 * p->b->natm;
   if (!p->gslots_d) HIPCHK(dalloc(&p->gslots_d, (size_t)GRAD_SLOTS * n));
   HIPCHK(hipMemsetAsync(p->gslots_d, 0, sizeof(double) * GRAD_SLOTS * n, stream));
-  for (const Bucket& bk : p->buckets) {
-    if (bk.mode == 1) {
-      set_err("mcint_plan_execute_grad: internal: derivative bucket on the single-kernel path");
-      return MCINT_ERR_UNSUPPORTED;
-    }
+  auto launch = [&](const Bucket& bk, hipStream_t s, const ScratchView& S) -> int {
     BucketArgs A = bucket_args(p, bk, nullptr, nullptr);
     A.dm = dm_dev;
     A.grad = p->gslots_d;
     A.nao = nao;
     A.nslot = GRAD_SLOTS;
     A.grad_stride = n;
-    const int64_t chunk = std::max<int64_t>(1, slice_tasks(bk, plan_scratch(p)));
+    A.scratch = S.blocks;
+    A.flags = S.flags;
+    const int64_t chunk = std::max<int64_t>(1, slice_tasks(bk, S));
     for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
       A.t0 = t;
       A.t1 = std::min(bk.t1, t + chunk);
       if (bk.mode == 2)
-        HIPCHK(launch_chunked(B, A, stream));
+        HIPCHK(launch_chunked(B, A, s));
       else
-        HIPCHK(launch_generic(B, A, stream));
+        HIPCHK(launch_generic(B, A, s));
     }
+    return MCINT_OK;
+  };
+  if (nbr > 1) {
+    if (const int st = launch_branched(p, stream, nbr, launch)) return st;
+  } else {
+    for (const Bucket& bk : p->buckets)
+      if (const int st = launch(bk, stream, plan_scratch(p))) return st;
   }
   HIPCHK(launch_grad_reduce(p->gslots_d, GRAD_SLOTS, n, grad_dev, stream));
   return MCINT_OK;
@@ -1489,7 +1504,10 @@ extern "C" int mcint_plan_execute(mcint_plan* p, double* out_dev, FINT* not0_dev
     p->gexec = nullptr;
     p->graph = nullptr;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    int st = nbr > 1 ? plan_launch_branched(p, out_dev, not0_dev, stream, nbr)
+    int st = nbr > 1 ? launch_branched(p, stream, nbr,
+                                       [&](const Bucket& bk, hipStream_t s, const ScratchView& S) {
+                                         return bucket_launch(p, bk, out_dev, not0_dev, s, S);
+                                       })
                      : plan_launch(p, out_dev, not0_dev, stream, plan_scratch(p));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
