@@ -154,14 +154,15 @@ def gradient_section(reps: int = 3) -> dict:
     B = mc.Basis(m.atm, m.bas, m.env)
     dm = random_density(m.nao_sph(), seed=5)
 
-    def timed(fn):
+    def timed(fn, reps_=None):
+        n = reps if reps_ is None else reps_
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(reps):
+        for _ in range(n):
             r = fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / reps, r
+        return (time.perf_counter() - t0) / n, r
 
     db = mc.DeviceBasis(B)
     q = unique_quartets(m.nbas)
@@ -200,6 +201,26 @@ def gradient_section(reps: int = 3) -> dict:
     t_grad32, g32 = timed(lambda: gp32.run(dm))
     g32_err = float((g32 - g).abs().max())
     gp32.close()
+    # the plain FP64 GPU path for the same work: every quartet on the
+    # generic kernel (one workgroup per quartet, runtime class, no register
+    # kernels, no FP32 tier); one run each
+    old = mc.tune(mc.TUNE_GENERIC_ONLY, 1)
+    try:
+        plan = mc.Plan(db, "int2e_sph", q)
+        out = torch.empty(plan.out_size, dtype=torch.float64, device="cuda")
+        t_plain_gen, _ = timed(lambda: plan.execute(out.data_ptr(), graph=False), reps_=1)
+        plan.close()
+        del out
+        jbg = jk.JKBuilder(B)
+        t_jk_gen, (vjg, vkg) = timed(lambda: jbg.run(dm), reps_=1)
+        jbg.close()
+        gpg = jk.GradientPlan(B, m.natm)
+        t_grad_gen, gg = timed(lambda: gpg.run(dm), reps_=1)
+        gpg.close()
+    finally:
+        mc.tune(mc.TUNE_GENERIC_ONLY, old)
+    gen_err = max(float((vjg - vj).abs().max()), float((vkg - vk).abs().max()),
+                  float((gg - g).abs().max()))
     db.close()
     return {
         "workload": f"c3: {m.name}, {m.natm} atoms, {m.nbas} shells, {m.nao_sph()} AOs; "
@@ -222,6 +243,22 @@ def gradient_section(reps: int = 3) -> dict:
         "jk_mixed_max_abs_dJ": dj,
         "jk_mixed_max_abs_dK": dk,
         "jk_mixed_energy_error": e32 - e64,
+        # north-star figures on this molecule, all for the same work: one
+        # int2e_sph pass over the unique quartets + one J/K build + one
+        # nuclear-gradient digest
+        "north_star": {
+            "batched_mixed_ms": (t_plain + t_jk32 + t_grad32) * 1e3,
+            "batched_fp64_ms": (t_plain + t_jk + t_grad) * 1e3,
+            "plain_fp64_generic_ms": (t_plain_gen + t_jk_gen + t_grad_gen) * 1e3,
+            "speedup_batched_mixed_vs_plain_fp64": (t_plain_gen + t_jk_gen + t_grad_gen)
+                                                   / (t_plain + t_jk32 + t_grad32),
+            "speedup_mixed_vs_fp64_batched": (t_plain + t_jk + t_grad) / (t_plain + t_jk32 + t_grad32),
+            "plain_fp64_generic_max_abs_diff_jk_grad": gen_err,
+            "definition": "plain FP64 GPU path = every quartet on the generic kernel (one "
+                          "workgroup per quartet, runtime class, FP64); batched mixed = class "
+                          "buckets on the register kernels in concurrent graph branches with "
+                          "the FP32 tier for Schwarz x max|D|^2 < 1e-7",
+        },
     }
 
 
