@@ -39,7 +39,8 @@ def lib():
                      "orc_int2e_ip1_cart", "orc_int3c2e_sph", "orc_int3c2e_cart",
                      "orc_int3c2e_ip1_sph", "orc_int3c2e_ip1_cart", "orc_int3c2e_ip2_sph",
                      "orc_int3c2e_ip2_cart", "orc_int2c2e_sph", "orc_int2c2e_cart",
-                     "orc_int2e_ip2_sph", "orc_int2e_ip2_cart"):
+                     "orc_int2e_ip2_sph", "orc_int2e_ip2_cart", "orc_int2e_ipip1_sph",
+                     "orc_int2e_ipip1_cart", "orc_int2e_ip1ip2_sph", "orc_int2e_ip1ip2_cart"):
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp]
@@ -122,12 +123,13 @@ class Basis:
 _KIND = {"int2e_sph": 0, "int2e_cart": 1, "int2e_ip1_sph": 2, "int2e_ip1_cart": 3,
          "int3c2e_sph": 4, "int3c2e_cart": 5, "int3c2e_ip1_sph": 6, "int3c2e_ip1_cart": 7,
          "int3c2e_ip2_sph": 8, "int3c2e_ip2_cart": 9, "int2c2e_sph": 10, "int2c2e_cart": 11,
-         "int2e_ip2_sph": 12, "int2e_ip2_cart": 13}
+         "int2e_ip2_sph": 12, "int2e_ip2_cart": 13, "int2e_ipip1_sph": 14,
+         "int2e_ipip1_cart": 15, "int2e_ip1ip2_sph": 16, "int2e_ip1ip2_cart": 17}
 
 
 def _shape(name):
     """(shell slots of shls used, derivative components) of an integral."""
-    ncomp = 3 if "_ip" in name else 1
+    ncomp = 9 if ("_ipip" in name or "_ip1ip2" in name) else (3 if "_ip" in name else 1)
     if name.startswith("int2c2e"):
         return (0, 1), ncomp
     if name.startswith("int3c2e"):

@@ -505,7 +505,9 @@ static void oc_strides(OcEnv *e)
 
 /* Four-centre (ij|kl), src/g2e.c:21-172; deriv_i raises i for int2e_ip1
  * (ng = {1,0,0,0,...}, src/autocode/grad2.c:69-88), deriv_k raises k for
- * int2e_ip2 (ng = {0,0,1,0,...}, src/autocode/grad2.c:151-170). */
+ * int2e_ip2 (ng = {0,0,1,0,...}, src/autocode/grad2.c:151-170); int2e_ipip1
+ * raises i by two (ng = {2,0,0,0,...}, src/autocode/hess.c:1175-1194) and
+ * int2e_ip1ip2 raises i and k by one (ng = {1,0,1,0,...}, hess.c:1339-1351). */
 static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas,
                        const double *env, int deriv_i, int deriv_k)
 {
@@ -520,7 +522,9 @@ static void oc_init_2e(OcEnv *e, const int *shls, const int *atm, const int *bas
         e->deriv_k = deriv_k;
         e->lc[0] += deriv_i;
         e->lc[2] += deriv_k;
-        e->ncomp = (deriv_i || deriv_k) ? 3 : 1;
+        /* second derivatives (ipip1: deriv_i = 2; ip1ip2: deriv_i = deriv_k = 1)
+         * have nine components */
+        e->ncomp = deriv_i + deriv_k == 2 ? 9 : ((deriv_i || deriv_k) ? 3 : 1);
         e->nfall = e->nf[0] * e->nf[1] * e->nf[2] * e->nf[3];
         e->common_factor = (OC_PI * OC_PI * OC_PI) * 2 / OC_SQRTPI *
                            common_fac_sp(e->l[0]) * common_fac_sp(e->l[1]) *
@@ -791,8 +795,11 @@ static void oc_gout_plain(double *gout, const double *g, const int *idx, const O
         }
 }
 
-/* nabla_i of g: f(i) = i g(i-1) - 2 ai g(i+1), src/g2e.c:4550-4591 */
-static void oc_nabla_i(double *f, const double *g, const OcEnv *e, double ai)
+/* nabla_i of g: f(i) = i g(i-1) - 2 ai g(i+1), src/g2e.c:4550-4591, for
+ * i <= li and k <= lk (the second derivatives take the first one a step
+ * further on the shell differentiated again: G2E_D_I/G2E_D_K with i_l+1,
+ * src/autocode/hess.c:1134-1136,1302-1304) */
+static void oc_nabla_i_to(double *f, const double *g, const OcEnv *e, double ai, int li, int lk)
 {
         int nr = e->nroots;
         double ai2 = -2 * ai;
@@ -801,8 +808,8 @@ static void oc_nabla_i(double *f, const double *g, const OcEnv *e, double ai)
                 double *fd = f + d * e->gsize;
                 for (int j = 0; j <= e->l[1]; j++)
                 for (int l = 0; l <= e->l[3]; l++)
-                for (int k = 0; k <= e->l[2]; k++)
-                for (int i = 0; i <= e->l[0]; i++)
+                for (int k = 0; k <= lk; k++)
+                for (int i = 0; i <= li; i++)
                 for (int r = 0; r < nr; r++) {
                         int p = r + i * e->gsi + k * e->gsk + l * e->gsl + j * e->gsj;
                         double v = ai2 * gd[p + e->gsi];
@@ -812,9 +819,15 @@ static void oc_nabla_i(double *f, const double *g, const OcEnv *e, double ai)
         }
 }
 
+static void oc_nabla_i(double *f, const double *g, const OcEnv *e, double ai)
+{
+        oc_nabla_i_to(f, g, e, ai, e->l[0], e->l[2]);
+}
+
 /* nabla on k of a four-centre g: f(k) = k g(k-1) - 2 ak g(k+1)
- * (CINTnabla1k_2e, src/g2e.c:4651-4697; G2E_D_K of src/autocode/grad2.c:107) */
-static void oc_nabla_k(double *f, const double *g, const OcEnv *e, double ak)
+ * (CINTnabla1k_2e, src/g2e.c:4651-4697; G2E_D_K of src/autocode/grad2.c:107),
+ * for i <= li and k <= lk */
+static void oc_nabla_k_to(double *f, const double *g, const OcEnv *e, double ak, int li, int lk)
 {
         int nr = e->nroots;
         double ak2 = -2 * ak;
@@ -823,13 +836,42 @@ static void oc_nabla_k(double *f, const double *g, const OcEnv *e, double ak)
                 double *fd = f + d * e->gsize;
                 for (int j = 0; j <= e->l[1]; j++)
                 for (int l = 0; l <= e->l[3]; l++)
-                for (int k = 0; k <= e->l[2]; k++)
-                for (int i = 0; i <= e->l[0]; i++)
+                for (int k = 0; k <= lk; k++)
+                for (int i = 0; i <= li; i++)
                 for (int r = 0; r < nr; r++) {
                         int p = r + i * e->gsi + k * e->gsk + l * e->gsl + j * e->gsj;
                         double v = ak2 * gd[p + e->gsk];
                         if (k > 0) v += k * gd[p - e->gsk];
                         fd[p] = v;
+                }
+        }
+}
+
+static void oc_nabla_k(double *f, const double *g, const OcEnv *e, double ak)
+{
+        oc_nabla_k_to(f, g, e, ak, e->l[0], e->l[2]);
+}
+
+/* Nine components of a second derivative, src/autocode/hess.c:1125-1174
+ * (int2e_ipip1) and :1290-1338 (int2e_ip1ip2): component 3p+q carries the
+ * first operator along axis p and the second along q; fa is the first
+ * operator applied to g, fb the second applied to g, fab the first applied
+ * to fb (axis p = q).  gout[9n + 3p + q]. */
+static void oc_gout_hess(double *gout, const double *g, const double *fa, const double *fb,
+                         const double *fab, const int *idx, const OcEnv *e)
+{
+        int nr = e->nroots;
+        for (int n = 0; n < e->nfall; n++) {
+                int ix[3] = {idx[3 * n], idx[3 * n + 1], idx[3 * n + 2]};
+                for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) {
+                        const double *src[3];
+                        for (int d = 0; d < 3; d++)
+                                src[d] = d == p ? (d == q ? fab : fa) : (d == q ? fb : g);
+                        double s = 0;
+                        for (int r = 0; r < nr; r++)
+                                s += src[0][ix[0] + r] * src[1][ix[1] + r] * src[2][ix[2] + r];
+                        gout[9 * n + 3 * p + q] += s;
                 }
         }
 }
@@ -1012,6 +1054,9 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const 
         double *gout = malloc(sizeof(double) * nfall * ncomp);
         double *g = malloc(sizeof(double) * e.gsize * 3);
         double *f = malloc(sizeof(double) * e.gsize * 3);
+        const int hess = ncomp == 9;
+        double *fb = hess ? calloc((size_t)e.gsize * 3, sizeof(double)) : NULL;
+        double *fab = hess ? calloc((size_t)e.gsize * 3, sizeof(double)) : NULL;
         int *idx = malloc(sizeof(int) * nfall * 3);
         int npij = nprim[0] * nprim[1], npkl = nprim[2] * nprim[3];
         double *cce_ij = malloc(sizeof(double) * npij * 5);
@@ -1091,7 +1136,17 @@ static int oc_int2e_drv(double *out, const int *dims, const int *shls_in, const 
                                 if (!st) continue;
                                 not0 = 1;
                                 memset(gout, 0, sizeof(double) * nfall * ncomp);
-                                if (deriv_i) {
+                                if (hess) {
+                                        /* ipip1: fb = D_i g to i_l+1, fab = D_i fb;
+                                         * ip1ip2: fb = D_k g to i_l+1, fab = D_i fb */
+                                        if (deriv_k)
+                                                oc_nabla_k_to(fb, g, &e, ex[2][kp], e.l[0] + 1, e.l[2]);
+                                        else
+                                                oc_nabla_i_to(fb, g, &e, ex[0][ip], e.l[0] + 1, e.l[2]);
+                                        oc_nabla_i_to(fab, fb, &e, ex[0][ip], e.l[0], e.l[2]);
+                                        oc_nabla_i(f, g, &e, ex[0][ip]);
+                                        oc_gout_hess(gout, g, f, fb, fab, idx, &e);
+                                } else if (deriv_i) {
                                         oc_nabla_i(f, g, &e, ex[0][ip]);
                                         oc_gout_ip1(gout, g, f, idx, &e);
                                 } else if (deriv_k) {
@@ -1155,7 +1210,8 @@ done:;
                 }
         }
         free(blk); free(b1); free(b2);
-        free(gctr); free(gout); free(g); free(f); free(idx); free(cce_ij); free(cce_kl);
+        free(gctr); free(gout); free(g); free(f); free(fb); free(fab); free(idx); free(cce_ij);
+        free(cce_kl);
         return not0;
 }
 
@@ -1197,6 +1253,32 @@ int orc_int2e_ip2_cart(double *out, const int *dims, const int *shls, const int 
 {
         (void)natm; (void)nbas;
         return oc_int2e_drv(out, dims, shls, atm, bas, env, 0, 0, 0, 1);
+}
+/* (d2/dr2 i j|k l), src/autocode/hess.c:1179-1194 */
+int orc_int2e_ipip1_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                        const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 2, 1, 0, 0);
+}
+int orc_int2e_ipip1_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                         const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 2, 0, 0, 0);
+}
+/* (d/dr i j|d/dr k l), src/autocode/hess.c:1343-1358 */
+int orc_int2e_ip1ip2_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                         const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 1, 0, 1);
+}
+int orc_int2e_ip1ip2_cart(double *out, const int *dims, const int *shls, const int *atm, int natm,
+                          const int *bas, int nbas, const double *env)
+{
+        (void)natm; (void)nbas;
+        return oc_int2e_drv(out, dims, shls, atm, bas, env, 1, 0, 0, 1);
 }
 int orc_int3c2e_sph(double *out, const int *dims, const int *shls, const int *atm, int natm,
                     const int *bas, int nbas, const double *env)
@@ -1282,6 +1364,10 @@ int orc_batch(int kind, double *out, const int64_t *off, const int *shls, int64_
                 case 10: st = orc_int2c2e_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 12: st = orc_int2e_ip2_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 case 13: st = orc_int2e_ip2_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 14: st = orc_int2e_ipip1_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 15: st = orc_int2e_ipip1_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 16: st = orc_int2e_ip1ip2_sph(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
+                case 17: st = orc_int2e_ip1ip2_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 default: st = orc_int2c2e_cart(out + off[q], NULL, s, atm, natm, bas, nbas, env); break;
                 }
                 if (not0) not0[q] = st;

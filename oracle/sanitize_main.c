@@ -22,6 +22,12 @@ int orc_int2e_cart(double *, const int *, const int *, const int *, int, const i
                    const double *);
 int orc_int2e_ip1_sph(double *, const int *, const int *, const int *, int, const int *, int,
                       const double *);
+int orc_int2e_ip2_sph(double *, const int *, const int *, const int *, int, const int *, int,
+                      const double *);
+int orc_int2e_ipip1_sph(double *, const int *, const int *, const int *, int, const int *, int,
+                        const double *);
+int orc_int2e_ip1ip2_sph(double *, const int *, const int *, const int *, int, const int *, int,
+                         const double *);
 int orc_int3c2e_sph(double *, const int *, const int *, const int *, int, const int *, int,
                     const double *);
 int orc_int3c2e_ip1_sph(double *, const int *, const int *, const int *, int, const int *, int,
@@ -64,6 +70,9 @@ int main(int argc, char **argv)
                 {"int2e_sph", orc_int2e_sph, 4, 1, 1},
                 {"int2e_cart", orc_int2e_cart, 4, 1, 0},
                 {"int2e_ip1_sph", orc_int2e_ip1_sph, 4, 3, 1},
+                {"int2e_ip2_sph", orc_int2e_ip2_sph, 4, 3, 1},
+                {"int2e_ipip1_sph", orc_int2e_ipip1_sph, 4, 9, 1},
+                {"int2e_ip1ip2_sph", orc_int2e_ip1ip2_sph, 4, 9, 1},
                 {"int3c2e_sph", orc_int3c2e_sph, 3, 1, 1},
                 {"int3c2e_ip1_sph", orc_int3c2e_ip1_sph, 3, 3, 1},
                 {"int3c2e_ip2_sph", orc_int3c2e_ip2_sph, 3, 3, 1},

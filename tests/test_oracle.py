@@ -486,6 +486,63 @@ def test_ip2_finite_difference(orc):
                                        atol=2e-7 * max(1.0, np.abs(fd).max()), err_msg=str(q))
 
 
+@pytest.mark.parametrize("name,first,slot", [("int2e_ipip1_cart", "int2e_ip1_cart", 0),
+                                              ("int2e_ipip1_sph", "int2e_ip1_sph", 0),
+                                              ("int2e_ip1ip2_cart", "int2e_ip1_cart", 2),
+                                              ("int2e_ip1ip2_sph", "int2e_ip1_sph", 2)])
+def test_second_derivatives_finite_difference(orc, name, first, slot):
+    """Component 3p+q of int2e_ipip1 = (d2/dr_p dr_q i j|k l) is -d/dA_q of
+    component p of int2e_ip1 (shell i moved), and of int2e_ip1ip2 =
+    (d/dr_p i j|d/dr_q k l) it is -d/dC_q of the same (shell k moved); central
+    differences (h = 1e-4) of the ip1 integrals, which the cint2e_ip1_sph
+    checksum pins (src/autocode/hess.c:1125-1174, 1290-1338)."""
+    from libcint_amd.synthetic import make_molecule
+    m = make_molecule("c3")
+    rng = np.random.default_rng(41 + slot + (name.endswith("sph")))
+    h = 1e-4
+    done = 0
+    while done < 5:
+        q = tuple(int(v) for v in rng.integers(0, m.nbas, 4))
+        if q[slot] in [q[s] for s in range(4) if s != slot]:
+            continue
+        done += 1
+        atm, bas, env, pc = _own_atom(m, q[slot])
+        g, _ = orc.intor(name, q, orc.Basis(atm, bas, env))
+        assert g.shape[-1] == 9
+        for d in range(3):
+            ep, em = env.copy(), env.copy()
+            ep[pc + d] += h
+            em[pc + d] -= h
+            vp, _ = orc.intor(first, q, orc.Basis(atm, bas, ep))
+            vm, _ = orc.intor(first, q, orc.Basis(atm, bas, em))
+            fd = -(vp - vm) / (2 * h)
+            for p_ in range(3):
+                np.testing.assert_allclose(g[..., 3 * p_ + d], fd[..., p_], rtol=0,
+                                           atol=2e-7 * max(1.0, np.abs(fd).max()),
+                                           err_msg="%s %s p=%d q=%d" % (name, q, p_, d))
+
+
+def test_second_derivative_symmetries(orc, fixture_basis):
+    """Over quartets of libcint's fixture: int2e_ipip1 is symmetric in its two
+    axes (component 3p+q = 3q+p), and (d_p i j|d_q k l) equals (d_q k l|d_p i j)
+    with the shell axes transposed (int2e_ip1ip2 of (k, l, i, j), component
+    3q+p), which fixes which operator the first index of a component names."""
+    B = orc.Basis(*fixture_basis)
+    rng = np.random.default_rng(8)
+    for _ in range(25):
+        i, j, k, l = (int(v) for v in rng.integers(0, 8, 4))
+        a, _ = orc.intor("int2e_ipip1_sph", (i, j, k, l), B)
+        scale = max(1.0, float(np.abs(a).max()))
+        a9 = a.reshape(a.shape[:4] + (3, 3))
+        np.testing.assert_allclose(a9, a9.swapaxes(4, 5), rtol=0, atol=1e-12 * scale)
+        c, _ = orc.intor("int2e_ip1ip2_sph", (i, j, k, l), B)
+        c_swap, _ = orc.intor("int2e_ip1ip2_sph", (k, l, i, j), B)
+        scale = max(1.0, float(np.abs(c).max()))
+        c9 = c.reshape(c.shape[:4] + (3, 3))
+        s9 = c_swap.reshape(c_swap.shape[:4] + (3, 3)).transpose(2, 3, 0, 1, 5, 4)
+        np.testing.assert_allclose(c9, s9, rtol=0, atol=1e-12 * scale)
+
+
 def test_ip2_electron_swap_and_translational_invariance(orc, fixture_basis):
     """Over quartets of libcint's fixture: (ij|d/dr k l) equals (d/dr k l|ij)
     with the axes transposed (int2e_ip1 of (k, l, i, j), pinned by the
