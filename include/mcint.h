@@ -76,6 +76,10 @@ typedef struct {
 #define MCINT_INT2C2E_CART 11
 #define MCINT_INT2E_IP2_SPH 12
 #define MCINT_INT2E_IP2_CART 13
+#define MCINT_INT2E_IPIP1_SPH 14
+#define MCINT_INT2E_IPIP1_CART 15
+#define MCINT_INT2E_IP1IP2_SPH 16
+#define MCINT_INT2E_IP1IP2_CART 17
 
 /* ---- 1. libcint-named per-call entry points --------------------------
  * Signature of libcint's CINTIntegralFunction (include/cint_funcs.h:10-17).
@@ -108,6 +112,20 @@ CACHE_SIZE_T int2e_ip2_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT 
                            FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
 CACHE_SIZE_T int2e_ip2_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
                             FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+/* (d2/dr2 i j|k l), 9 components (component 3p+q differentiates i along
+ * axes p and q): out (di,dj,dk,dl,9); replaces src/autocode/hess.c:1179-1194
+ * (cart 1179-1186, sph 1187-1194). */
+CACHE_SIZE_T int2e_ipip1_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                             FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int2e_ipip1_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                              FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+/* (d/dr i j|d/dr k l), 9 components (component 3p+q: i along axis p, k
+ * along axis q): out (di,dj,dk,dl,9); replaces src/autocode/hess.c:1343-1358
+ * (cart 1343-1350, sph 1351-1358). */
+CACHE_SIZE_T int2e_ip1ip2_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                              FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
+CACHE_SIZE_T int2e_ip1ip2_cart(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
+                               FINT *bas, FINT nbas, double *env, CINTOpt *opt, double *cache);
 /* (d/dr i j|k), 3 components, derivative on the first shell: out
  * (di,dj,dk,3); replaces src/autocode/int3c2e.c:72-87 (cart 72-79, sph 80-87). */
 CACHE_SIZE_T int3c2e_ip1_sph(double *out, FINT *dims, FINT *shls, FINT *atm, FINT natm,
@@ -133,6 +151,9 @@ void int2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, 
 void int2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 /* src/autocode/grad2.c:151-154 */
 void int2e_ip2_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+/* src/autocode/hess.c:1175-1178, 1339-1342 */
+void int2e_ipip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
+void int2e_ip1ip2_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 void int3c2e_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
 /* src/autocode/int3c2e.c:68-71, 149-152; src/cint2c2e.c:360-366 */
 void int3c2e_ip1_optimizer(CINTOpt **opt, FINT *atm, FINT natm, FINT *bas, FINT nbas, double *env);
@@ -235,6 +256,19 @@ int int2e_ip2_sph_batch(double *out, const size_t *out_offsets, const FINT *shls
 int int2e_ip2_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
                          const FINT *atm, FINT natm, const FINT *bas, FINT nbas, const double *env,
                          const CINTOpt *opt, FINT *not0, void *hip_stream);
+/* int2e_ipip1 and int2e_ip1ip2 over a list of quartets (blocks (di,dj,dk,dl,9)) */
+int int2e_ipip1_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                          const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                          const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_ipip1_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                           const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                           const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_ip1ip2_sph_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                           const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                           const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
+int int2e_ip1ip2_cart_batch(double *out, const size_t *out_offsets, const FINT *shls, size_t nq,
+                            const FINT *atm, FINT natm, const FINT *bas, FINT nbas,
+                            const double *env, const CINTOpt *opt, FINT *not0, void *hip_stream);
 
 /* ---- 3. device-resident plans ------------------------------------------- */
 typedef struct mcint_basis mcint_basis;

@@ -21,7 +21,8 @@ LIB_PATH = os.path.join(HERE, "libmcint.so")
 
 INTOR_NAMES = ("int2e_sph", "int2e_cart", "int2e_ip1_sph", "int2e_ip1_cart", "int3c2e_sph",
                "int3c2e_cart", "int3c2e_ip1_sph", "int3c2e_ip1_cart", "int3c2e_ip2_sph",
-               "int3c2e_ip2_cart", "int2c2e_sph", "int2c2e_cart", "int2e_ip2_sph", "int2e_ip2_cart")
+               "int3c2e_ip2_cart", "int2c2e_sph", "int2c2e_cart", "int2e_ip2_sph", "int2e_ip2_cart",
+               "int2e_ipip1_sph", "int2e_ipip1_cart", "int2e_ip1ip2_sph", "int2e_ip1ip2_cart")
 INTOR_CODES = {n: i for i, n in enumerate(INTOR_NAMES)}
 (INT2E_SPH, INT2E_CART, INT2E_IP1_SPH, INT2E_IP1_CART, INT3C2E_SPH, INT3C2E_CART, INT3C2E_IP1_SPH,
  INT3C2E_IP1_CART, INT3C2E_IP2_SPH, INT3C2E_IP2_CART, INT2C2E_SPH, INT2C2E_CART, INT2E_IP2_SPH,
@@ -30,7 +31,7 @@ INTOR_CODES = {n: i for i, n in enumerate(INTOR_NAMES)}
 
 def intor_shape(name: str):
     """(slots of a 4-int shell tuple the integral uses, derivative components)."""
-    ncomp = 3 if "_ip" in name else 1
+    ncomp = 9 if ("_ipip" in name or "_ip1ip2" in name) else (3 if "_ip" in name else 1)
     if name.startswith("int2c2e"):
         return (0, 1), ncomp
     if name.startswith("int3c2e"):
@@ -123,7 +124,7 @@ def lib():
     L.mcint_last_error.restype = ctypes.c_char_p
     for name in ("int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer",
                  "int3c2e_ip1_optimizer", "int3c2e_ip2_optimizer", "int2c2e_optimizer",
-                 "int2e_ip2_optimizer"):
+                 "int2e_ip2_optimizer", "int2e_ipip1_optimizer", "int2e_ip1ip2_optimizer"):
         f = getattr(L, name)
         f.restype = None
         f.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_int, ip, ctypes.c_int, dp]

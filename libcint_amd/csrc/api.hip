@@ -77,6 +77,10 @@ int intor_kind(int intor) {
     case MCINT_INT3C2E_IP2_CART: return KIND_3C2E_IP2;
     case MCINT_INT2C2E_SPH:
     case MCINT_INT2C2E_CART: return KIND_2C2E;
+    case MCINT_INT2E_IPIP1_SPH:
+    case MCINT_INT2E_IPIP1_CART: return KIND_2E_IPIP1;
+    case MCINT_INT2E_IP1IP2_SPH:
+    case MCINT_INT2E_IP1IP2_CART: return KIND_2E_IP1IP2;
     default: return -1;
   }
 }
@@ -297,7 +301,7 @@ int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, Q
     Q.np[a] = bs[NPRIM_OF];
     Q.w[a] = sph ? nsph(Q.l[a]) : ncart(Q.l[a]);
   }
-  const int lsum = Q.l[0] + Q.l[1] + Q.l[2] + Q.l[3] + (kind_ncomp(kind) == 3 ? 1 : 0);
+  const int lsum = Q.l[0] + Q.l[1] + Q.l[2] + Q.l[3] + kind_bra_raise(kind) + kind_ket_raise(kind);
   if (lsum / 2 + 1 > NRYS_MAX) return 2;
   Q.so[0] = 1;
   for (int o = 0; o < 4; ++o) Q.so[o + 1] = Q.so[o] * (int64_t)Q.w[o] * Q.nc[o];
@@ -319,12 +323,13 @@ int describe_tuple(const int* bas, int nbas, int kind, int sph, const FINT* s, Q
     L[2] = 0;
     L[3] = 1;
     if (l[0] < l[1]) std::swap(L[2], L[3]);
-  } else if (kind == KIND_2E_IP1) {
+  } else if (kind == KIND_2E_IP1 || kind == KIND_2E_IPIP1) {
     if (l[2] < l[3]) std::swap(L[2], L[3]);
   } else if (kind == KIND_3C2E || kind == KIND_3C2E_IP2) {
     if (l[0] < l[1]) std::swap(L[0], L[1]);
   }
-  // the derivative shell of int3c2e_ip1 stays first; int2c2e keeps (i, k)
+  // the derivative shell of int3c2e_ip1 stays first; int2c2e keeps (i, k);
+  // int2e_ip1ip2 keeps (i, j, k, l): both derivative shells stay in place
   return 0;
 }
 
@@ -458,8 +463,7 @@ constexpr int ncart_sum(int lo, int hi) {
 // kernel counted 15x (its measured slowdown).
 double bucket_cost(int kind, int mode, int la, int lb, int lc, int ld, int64_t nprim,
                    int64_t ntask) {
-  const int d = kind_ncomp(kind) == 3 ? 1 : 0;
-  const int E = la + lb + d, F = lc + ld;
+  const int E = la + lb + kind_bra_raise(kind), F = lc + ld + kind_ket_raise(kind);
   const int nr = (E + F) / 2 + 1;
   const double nef = (double)ncart_sum(la, E) * ncart_sum(lc, F);
   const double c = (double)nprim * nr * (2.0 * nef + 9.0 * (E + 1) * (F + 1)) + 4.0 * ntask * nef;
@@ -487,7 +491,7 @@ constexpr int MAX_GRAPH_BRANCHES = 8;
 // per-class overrides (mcint_tune_class), 0 = the global value; indexed by
 // integral kind and canonical class
 constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
-int16_t g_class_ppl[6][NCLS] = {};
+int16_t g_class_ppl[NKIND][NCLS] = {};
 
 inline int class_index(int la, int lb, int lc, int ld) {
   return ((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld;
@@ -886,9 +890,9 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
     } else if (bk.mode == 0) {
       const GenLayout G = gen_layout(kind, sph, bk.la, bk.lb, bk.lc, bk.ld);
       const int nr = (bk.la + bk.lb + bk.lc + (kind_three(kind) ? 0 : bk.ld) +
-                      (kind_ncomp(kind) == 3 ? 1 : 0)) / 2 + 1;
+                      kind_bra_raise(kind) + kind_ket_raise(kind)) / 2 + 1;
       const int nrt = b->omega < 0 ? 2 * nr : nr;
-      if (G.nE > 220 || G.nF > 165 || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
+      if (G.nE > GEN_ECAP || G.nF > GEN_FCAP || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
         plan_free(p);
         set_err("mcint_plan_create: angular momentum beyond the generic kernel's tables");
         return MCINT_ERR_UNSUPPORTED;
@@ -1667,6 +1671,10 @@ MC_BATCH(int2c2e_sph_batch, MCINT_INT2C2E_SPH)
 MC_BATCH(int2c2e_cart_batch, MCINT_INT2C2E_CART)
 MC_BATCH(int2e_ip2_sph_batch, MCINT_INT2E_IP2_SPH)
 MC_BATCH(int2e_ip2_cart_batch, MCINT_INT2E_IP2_CART)
+MC_BATCH(int2e_ipip1_sph_batch, MCINT_INT2E_IPIP1_SPH)
+MC_BATCH(int2e_ipip1_cart_batch, MCINT_INT2E_IPIP1_CART)
+MC_BATCH(int2e_ip1ip2_sph_batch, MCINT_INT2E_IP1IP2_SPH)
+MC_BATCH(int2e_ip1ip2_cart_batch, MCINT_INT2E_IP1IP2_CART)
 #undef MC_BATCH
 
 // ---------------------------------------------------------------------------
@@ -1890,9 +1898,10 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
     sstride = chunked_block_doubles(kind, la, lb, lc, ld);
   } else if (mode == 0) {
     const GenLayout G = gen_layout(kind, sph, la, lb, lc, ld);
-    const int nr = (la + lb + lc + ld + (ncomp == 3 ? 1 : 0)) / 2 + 1;
+    const int nr = (la + lb + lc + (kind_three(kind) ? 0 : ld) + kind_bra_raise(kind) +
+                    kind_ket_raise(kind)) / 2 + 1;
     const int nrt = b->omega < 0 ? 2 * nr : nr;
-    if (G.nE > 220 || G.nF > 165 || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
+    if (G.nE > GEN_ECAP || G.nF > GEN_FCAP || 3 * nrt * (G.E + 1) * (G.F + 1) > GEN_VCAP) {
       set_err("angular momentum beyond the generic kernel's tables");
       return -1;
     }
@@ -2059,6 +2068,10 @@ MC_INTOR(int2c2e_sph, MCINT_INT2C2E_SPH)
 MC_INTOR(int2c2e_cart, MCINT_INT2C2E_CART)
 MC_INTOR(int2e_ip2_sph, MCINT_INT2E_IP2_SPH)
 MC_INTOR(int2e_ip2_cart, MCINT_INT2E_IP2_CART)
+MC_INTOR(int2e_ipip1_sph, MCINT_INT2E_IPIP1_SPH)
+MC_INTOR(int2e_ipip1_cart, MCINT_INT2E_IPIP1_CART)
+MC_INTOR(int2e_ip1ip2_sph, MCINT_INT2E_IP1IP2_SPH)
+MC_INTOR(int2e_ip1ip2_cart, MCINT_INT2E_IP1IP2_CART)
 #undef MC_INTOR
 
 extern "C" void int2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
@@ -2072,6 +2085,14 @@ extern "C" void int2e_ip1_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* b
 extern "C" void int2e_ip2_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
                                     double* env) {
   make_opt(opt, KIND_2E_IP1, atm, natm, bas, nbas, env);
+}
+extern "C" void int2e_ipip1_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                      double* env) {
+  make_opt(opt, KIND_2E_IPIP1, atm, natm, bas, nbas, env);
+}
+extern "C" void int2e_ip1ip2_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
+                                       double* env) {
+  make_opt(opt, KIND_2E_IP1IP2, atm, natm, bas, nbas, env);
 }
 extern "C" void int3c2e_optimizer(CINTOpt** opt, FINT* atm, FINT natm, FINT* bas, FINT nbas,
                                   double* env) {

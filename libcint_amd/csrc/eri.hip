@@ -118,8 +118,6 @@ __device__ __forceinline__ OutStrides out_strides(const VShell* vsh, const Task&
 // per-task scratch block; used for every class without a fast kernel.
 // ---------------------------------------------------------------------
 constexpr int GEN_LANES = 64;
-constexpr int GEN_ECAP = 220;                // sum of ncart(l), l <= 9
-constexpr int GEN_FCAP = 165;                // l <= 8
 
 __device__ __forceinline__ double binom_small(int n, int k) {
   double r = 1;
@@ -155,55 +153,26 @@ __device__ double hrr_bra(const double* __restrict__ X, int nF, int f, int set_l
   return s;
 }
 
-__global__ void __launch_bounds__(GEN_LANES)
-eri_generic_kernel(BasisView B, BucketArgs A) {
-  __shared__ double s_u[2 * NRYS_MAX], s_w[2 * NRYS_MAX];
-  extern __shared__ double s_V[];  // 3 * roots * (E+1)(F+1), sized at launch
-  __shared__ int s_ec[GEN_ECAP];
-  __shared__ int s_fc[GEN_FCAP];
-
-  const int64_t t = A.t0 + blockIdx.x;
-  if (t >= A.t1) return;
+// Primitive-quartet walk shared by the generic kernels: for every
+// primitive quartet of task tk that passes libcint's screening
+// (src/cint2e.c:57-269 with the optimizer's cutoffs; short range as in
+// src/cint2e.c:358-380), the Rys roots and weights of the operator and the
+// 2D recurrence tables V_d(e, f), e <= E, f <= F, of each root r and
+// direction d into LDS at s_V + (d * nrt + r) * (E+1)(F+1); then
+// consume(ai, ak, nrt) with the exponents of primitives i and k (the
+// auxiliary shell's for three centres), after which the workgroup
+// synchronises.  lij and lkl are the angular sums of the bra and ket with
+// their derivative raises (the distance terms of the screening estimate).
+// Returns libcint's not0: whether any primitive quartet contributed.
+template <class Consume>
+__device__ int gen_walk(const BasisView& B, const Task& tk, const VShell& s0, const VShell& s1,
+                        const VShell& s2, const VShell& s3, int kind, int la, int lb, int lc,
+                        int ld, int lij, int lkl, int ER, int FR, double* s_u, double* s_w,
+                        double* s_V, Consume&& consume) {
   const int lane = threadIdx.x;
-  const Task tk = A.tasks[t];
-  const int kind = A.kind;
-  const bool ip1 = kind_bra_der(kind);
-  const bool ip2 = kind_ket_der(kind);
   const bool three = kind_three(kind);
   // (i|k): no screening at all (src/cint2c2e.c:57-147)
   const bool noscreen = kind == KIND_2C2E;
-  const int la = A.la, lb = A.lb, lc = A.lc, ld = three ? 0 : A.ld;
-  const GenLayout G = gen_layout(kind, A.sph, la, lb, lc, ld);
-  double* X = A.scratch + (t - A.t0) * A.scratch_stride;
-
-  const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
-  VShell s3 = s2;
-  if (!three) s3 = B.vsh[tk.sh[3]];
-  const OutStrides OS = out_strides(B.vsh, tk, task_perm(tk), three ? 3 : 4, A.sph);
-  double* out = A.out + task_out(tk);
-
-  // component tables of the union e-list and the f-list
-  for (int i = lane; i < G.nE; i += GEN_LANES) {
-    int l = G.e_lo, k = i;
-    while (k >= ncart(l)) { k -= ncart(l); ++l; }
-    // k-th component of shell l in libcint order
-    int m = 0;
-    while ((m + 1) * (m + 2) / 2 <= k) ++m;
-    const int lz = k - m * (m + 1) / 2;
-    const int lx = l - m;
-    const int ly = m - lz;
-    s_ec[i] = lx | (ly << 4) | (lz << 8) | (l << 12);
-  }
-  for (int i = lane; i < G.nF; i += GEN_LANES) {
-    int l = lc, k = i;
-    while (k >= ncart(l)) { k -= ncart(l); ++l; }
-    int m = 0;
-    while ((m + 1) * (m + 2) / 2 <= k) ++m;
-    const int lz = k - m * (m + 1) / 2;
-    s_fc[i] = (l - m) | ((m - lz) << 4) | (lz << 8) | (l << 12);
-  }
-  for (int64_t i = lane; i < G.oH1; i += GEN_LANES) X[i] = 0;
-
   const int nij = s0.nprim * s1.nprim;
   const int nkl = three ? s2.nprim : s2.nprim * s3.nprim;
   // records visited in libcint's order (ip fastest), so that the sums over
@@ -217,8 +186,6 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
   const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
-  const int lij = la + lb + (ip1 ? 1 : 0);
-  const int lkl = lc + ld + (ip2 ? 1 : 0);
   const int nr = (lij + lkl) / 2 + 1;  // rys_order
   const double omega = B.omega;
   const bool sr = omega < 0;
@@ -262,9 +229,9 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   // src/g2e.c:4455-4476, and a dedicated rule on [sqrt(theta), 1] above;
   // both integrate the polynomial integrand exactly)
   const int nrt = sr ? 2 * nr : nr;
-  const int VS = (G.E + 1) * (G.F + 1);
-  const double cfac = PI3_2_SQRTPI * common_fac_sp(A.la) * common_fac_sp(A.lb) *
-                      common_fac_sp(A.lc) * (three ? 1.0 : common_fac_sp(ld));
+  const int VS = (ER + 1) * (FR + 1);
+  const double cfac = PI3_2_SQRTPI * common_fac_sp(la) * common_fac_sp(lb) *
+                      common_fac_sp(lc) * (three ? 1.0 : common_fac_sp(ld));
   const double Ax = s0.r[0], Ay = s0.r[1], Az = s0.r[2];
   const double Cx = s2.r[0], Cy = s2.r[1], Cz = s2.r[2];
   const double* ek = B.env + s2.pexp;
@@ -272,7 +239,6 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   int any = 0;
 
   if (live) {
-    const int ncomp_g = G.nE * G.nF;
     for (int p = 0; p < nij * nkl; ++p) {
       const int ij = p % nij;
       const int kl = p / nij;
@@ -348,15 +314,15 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         const double c00 = pa - tmp1 * akl * pq;
         const double c0p = qc + tmp1 * aij * pq;
         double* V = s_V + (d * nrt + r) * VS;
-        const int F1 = G.F + 1;
+        const int F1 = FR + 1;
         V[0] = d == 2 ? s_w[r] : 1.0;
-        for (int e = 0; e < G.E; ++e) {
+        for (int e = 0; e < ER; ++e) {
           double v = c00 * V[e * F1];
           if (e > 0) v += e * b10 * V[(e - 1) * F1];
           V[(e + 1) * F1] = v;
         }
-        for (int f = 0; f < G.F; ++f) {
-          for (int e = 0; e <= G.E; ++e) {
+        for (int f = 0; f < FR; ++f) {
+          for (int e = 0; e <= ER; ++e) {
             double v = c0p * V[e * F1 + f];
             if (f > 0) v += f * b01 * V[e * F1 + f - 1];
             if (e > 0) v += e * b00 * V[(e - 1) * F1 + f];
@@ -365,65 +331,122 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
         }
       }
       __syncthreads();
-      const double wai = 2.0 * P.ai;
-      if (ip2) {
-        // (e | d/dr_comp k) = k_comp [e|k-1_comp] - 2 a_k [e|k+1_comp], per
-        // primitive quartet (the auxiliary exponent a_k is akl here)
-        const int F1 = G.F + 1;
-        const double mak2 = -2.0 * akl;
-        for (int c = lane; c < 3 * ncomp_g; c += GEN_LANES) {
-          const int comp = c / ncomp_g;
-          const int ie = (c - comp * ncomp_g) / G.nF;
-          const int jf = c - comp * ncomp_g - ie * G.nF;
-          const int ec = s_ec[ie], fc = s_fc[jf];
-          const int e3[3] = {ec & 15, (ec >> 4) & 15, (ec >> 8) & 15};
-          const int f3[3] = {fc & 15, (fc >> 4) & 15, (fc >> 8) & 15};
-          double g = 0;
-          for (int r = 0; r < nrt; ++r) {
-            double prod = 1.0;
-            for (int d = 0; d < 3; ++d) {
-              const double* V = s_V + (d * nrt + r) * VS + e3[d] * F1;
-              double v;
-              if (d == comp) {
-                v = mak2 * V[f3[d] + 1];
-                if (f3[d] > 0) v += f3[d] * V[f3[d] - 1];
-              } else {
-                v = V[f3[d]];
-              }
-              prod *= v;
-            }
-            g += prod;
-          }
-          X[G.oX0 + ((int64_t)comp * G.NE0 + ie) * G.nF + jf] += g;
-        }
-        __syncthreads();
-        continue;
-      }
-      for (int c = lane; c < ncomp_g; c += GEN_LANES) {
-        const int ie = c / G.nF;
-        const int jf = c - ie * G.nF;
-        const int ec = s_ec[ie], fc = s_fc[jf];
-        const int ex = ec & 15, ey = (ec >> 4) & 15, ez = (ec >> 8) & 15, le = ec >> 12;
-        const int fx = fc & 15, fy = (fc >> 4) & 15, fz = (fc >> 8) & 15;
-        const int F1 = G.F + 1;
-        double g = 0;
-        for (int r = 0; r < nrt; ++r) {
-          g += s_V[(0 * nrt + r) * VS + ex * F1 + fx] * s_V[(1 * nrt + r) * VS + ey * F1 + fy] *
-               s_V[(2 * nrt + r) * VS + ez * F1 + fz];
-        }
-        if (le >= G.p_lo && le <= G.p_hi) {
-          const int e0 = cart_sum(G.p_lo, le - 1) + cart_idx(le, ex, ez);
-          X[G.oX0 + (int64_t)e0 * G.nF + jf] += ip1 ? wai * g : g;
-        }
-        if (le >= G.m_lo && le <= G.m_hi) {
-          const int e1 = cart_sum(G.m_lo, le - 1) + cart_idx(le, ex, ez);
-          X[G.oXM + (int64_t)e1 * G.nF + jf] += g;
-        }
-      }
+      consume(P.ai, three ? akl : pkl[pos_kl[kl]].ai, nrt);
       __syncthreads();
     }
   }
-  any = __syncthreads_or(any) && live;
+  return __syncthreads_or(any) && live;
+}
+
+__global__ void __launch_bounds__(GEN_LANES)
+eri_generic_kernel(BasisView B, BucketArgs A) {
+  __shared__ double s_u[2 * NRYS_MAX], s_w[2 * NRYS_MAX];
+  extern __shared__ double s_V[];  // 3 * roots * (E+1)(F+1), sized at launch
+  __shared__ int s_ec[GEN_ECAP];
+  __shared__ int s_fc[GEN_FCAP];
+
+  const int64_t t = A.t0 + blockIdx.x;
+  if (t >= A.t1) return;
+  const int lane = threadIdx.x;
+  const Task tk = A.tasks[t];
+  const int kind = A.kind;
+  const bool ip1 = kind_bra_der(kind);
+  const bool ip2 = kind_ket_der(kind);
+  const bool three = kind_three(kind);
+  const int la = A.la, lb = A.lb, lc = A.lc, ld = three ? 0 : A.ld;
+  const GenLayout G = gen_layout(kind, A.sph, la, lb, lc, ld);
+  double* X = A.scratch + (t - A.t0) * A.scratch_stride;
+
+  const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
+  VShell s3 = s2;
+  if (!three) s3 = B.vsh[tk.sh[3]];
+  const OutStrides OS = out_strides(B.vsh, tk, task_perm(tk), three ? 3 : 4, A.sph);
+  double* out = A.out + task_out(tk);
+
+  // component tables of the union e-list and the f-list
+  for (int i = lane; i < G.nE; i += GEN_LANES) {
+    int l = G.e_lo, k = i;
+    while (k >= ncart(l)) { k -= ncart(l); ++l; }
+    // k-th component of shell l in libcint order
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= k) ++m;
+    const int lz = k - m * (m + 1) / 2;
+    const int lx = l - m;
+    const int ly = m - lz;
+    s_ec[i] = lx | (ly << 4) | (lz << 8) | (l << 12);
+  }
+  for (int i = lane; i < G.nF; i += GEN_LANES) {
+    int l = lc, k = i;
+    while (k >= ncart(l)) { k -= ncart(l); ++l; }
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= k) ++m;
+    const int lz = k - m * (m + 1) / 2;
+    s_fc[i] = (l - m) | ((m - lz) << 4) | (lz << 8) | (l << 12);
+  }
+  for (int64_t i = lane; i < G.oH1; i += GEN_LANES) X[i] = 0;
+
+  const int ncomp_g = G.nE * G.nF;
+  const int VS = (G.E + 1) * (G.F + 1);
+  auto consume = [&](double ai, double akl, int nrt) {
+    const double wai = 2.0 * ai;
+    if (ip2) {
+      // (e | d/dr_comp k) = k_comp [e|k-1_comp] - 2 a_k [e|k+1_comp], per
+      // primitive quartet (the auxiliary exponent a_k is akl here)
+      const int F1 = G.F + 1;
+      const double mak2 = -2.0 * akl;
+      for (int c = lane; c < 3 * ncomp_g; c += GEN_LANES) {
+        const int comp = c / ncomp_g;
+        const int ie = (c - comp * ncomp_g) / G.nF;
+        const int jf = c - comp * ncomp_g - ie * G.nF;
+        const int ec = s_ec[ie], fc = s_fc[jf];
+        const int e3[3] = {ec & 15, (ec >> 4) & 15, (ec >> 8) & 15};
+        const int f3[3] = {fc & 15, (fc >> 4) & 15, (fc >> 8) & 15};
+        double g = 0;
+        for (int r = 0; r < nrt; ++r) {
+          double prod = 1.0;
+          for (int d = 0; d < 3; ++d) {
+            const double* V = s_V + (d * nrt + r) * VS + e3[d] * F1;
+            double v;
+            if (d == comp) {
+              v = mak2 * V[f3[d] + 1];
+              if (f3[d] > 0) v += f3[d] * V[f3[d] - 1];
+            } else {
+              v = V[f3[d]];
+            }
+            prod *= v;
+          }
+          g += prod;
+        }
+        X[G.oX0 + ((int64_t)comp * G.NE0 + ie) * G.nF + jf] += g;
+      }
+      return;
+    }
+    for (int c = lane; c < ncomp_g; c += GEN_LANES) {
+      const int ie = c / G.nF;
+      const int jf = c - ie * G.nF;
+      const int ec = s_ec[ie], fc = s_fc[jf];
+      const int ex = ec & 15, ey = (ec >> 4) & 15, ez = (ec >> 8) & 15, le = ec >> 12;
+      const int fx = fc & 15, fy = (fc >> 4) & 15, fz = (fc >> 8) & 15;
+      const int F1 = G.F + 1;
+      double g = 0;
+      for (int r = 0; r < nrt; ++r) {
+        g += s_V[(0 * nrt + r) * VS + ex * F1 + fx] * s_V[(1 * nrt + r) * VS + ey * F1 + fy] *
+             s_V[(2 * nrt + r) * VS + ez * F1 + fz];
+      }
+      if (le >= G.p_lo && le <= G.p_hi) {
+        const int e0 = cart_sum(G.p_lo, le - 1) + cart_idx(le, ex, ez);
+        X[G.oX0 + (int64_t)e0 * G.nF + jf] += ip1 ? wai * g : g;
+      }
+      if (le >= G.m_lo && le <= G.m_hi) {
+        const int e1 = cart_sum(G.m_lo, le - 1) + cart_idx(le, ex, ez);
+        X[G.oXM + (int64_t)e1 * G.nF + jf] += g;
+      }
+    }
+  };
+  const int lij = la + lb + (ip1 ? 1 : 0);
+  const int lkl = lc + ld + (ip2 ? 1 : 0);
+  const int any = gen_walk(B, tk, s0, s1, s2, s3, kind, la, lb, lc, ld, lij, lkl, G.E, G.F, s_u,
+                           s_w, s_V, consume);
   if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
 
   if (!any) {
@@ -609,6 +632,252 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------
+// Second-derivative kernel (int2e_ipip1, int2e_ip1ip2): one 64-lane
+// workgroup per contracted quartet, runtime class.  The derivative of a
+// Cartesian Gaussian of exponent a along axis p is
+//   D_p |a> = a_p |a - 1_p> - 2a |a + 1_p>        (src/g2e.c:4550-4697),
+// so primitive sums are kept in row sets weighted by powers of 2a and the
+// derivatives are formed after contraction, before the transfers:
+//   ipip1  (bra sets la-2.., la.., la+2.. weighted 1, 2a_i, 4a_i^2)
+//     D_p D_q |a> = a_p a_q |a-1_p-1_q> - a_q 2a|a+1_p-1_q> - a_p 2a|a-1_p+1_q>
+//                   + 4a^2 |a+1_p+1_q>                               (p != q)
+//     D_p D_p |a> = a_p (a_p-1) |a-2_p> - (2a_p+1) 2a|a> + 4a^2 |a+2_p>
+//   ip1ip2 (bra sets la-1.., la+1.. weighted 1, 2a_i; ket sets lc-1..,
+//     lc+1.. weighted 1, 2a_k): D_p on the bra, D_q on the ket.
+// Component 3p+q of the output as libcint orders it
+// (src/autocode/hess.c:1125-1174, 1289-1338).
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(GEN_LANES)
+eri_hess_kernel(BasisView B, BucketArgs A) {
+  __shared__ double s_u[2 * NRYS_MAX], s_w[2 * NRYS_MAX];
+  extern __shared__ double s_V[];  // 3 * roots * (E+1)(F+1), sized at launch
+  __shared__ int s_ec[GEN_ECAP];
+  __shared__ int s_fc[GEN_FCAP];
+
+  const int64_t t = A.t0 + blockIdx.x;
+  if (t >= A.t1) return;
+  const int lane = threadIdx.x;
+  const Task tk = A.tasks[t];
+  const int kind = A.kind;
+  const bool ipip = kind == KIND_2E_IPIP1;
+  const int la = A.la, lb = A.lb, lc = A.lc, ld = A.ld;
+  const GenLayout G = gen_layout(kind, A.sph, la, lb, lc, ld);
+  double* X = A.scratch + (t - A.t0) * A.scratch_stride;
+  const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]],
+               s3 = B.vsh[tk.sh[3]];
+  const OutStrides OS = out_strides(B.vsh, tk, task_perm(tk), 4, A.sph);
+  double* out = A.out + task_out(tk);
+
+  // Cartesian exponents of the union e-list (shells e_lo..E) and f-list
+  for (int i = lane; i < G.nE + G.nF; i += GEN_LANES) {
+    const bool isf = i >= G.nE;
+    int l = isf ? G.f_lo : G.e_lo, k = isf ? i - G.nE : i;
+    while (k >= ncart(l)) { k -= ncart(l); ++l; }
+    int m = 0;
+    while ((m + 1) * (m + 2) / 2 <= k) ++m;
+    const int lz = k - m * (m + 1) / 2;
+    const int code = (l - m) | ((m - lz) << 4) | (lz << 8) | (l << 12);
+    if (isf) s_fc[i - G.nE] = code;
+    else s_ec[i] = code;
+  }
+  for (int64_t i = lane; i < G.oH1; i += GEN_LANES) X[i] = 0;
+
+  const int VS = (G.E + 1) * (G.F + 1);
+  const int F1 = G.F + 1;
+  auto consume = [&](double ai, double ak, int nrt) {
+    const double wb[3] = {1.0, 2.0 * ai, 4.0 * ai * ai};
+    const double wk[2] = {1.0, 2.0 * ak};
+    for (int c = lane; c < G.nE * G.nF; c += GEN_LANES) {
+      const int ie = c / G.nF;
+      const int jf = c - ie * G.nF;
+      const int ec = s_ec[ie], fc = s_fc[jf];
+      const int ex = ec & 15, ey = (ec >> 4) & 15, ez = (ec >> 8) & 15, le = ec >> 12;
+      const int fx = fc & 15, fy = (fc >> 4) & 15, fz = (fc >> 8) & 15, lf = fc >> 12;
+      double g = 0;
+      for (int r = 0; r < nrt; ++r) {
+        g += s_V[(0 * nrt + r) * VS + ex * F1 + fx] * s_V[(1 * nrt + r) * VS + ey * F1 + fy] *
+             s_V[(2 * nrt + r) * VS + ez * F1 + fz];
+      }
+      for (int u = 0; u < 2; ++u) {
+        if (G.kset_lo[u] < 0 || lf < G.kset_lo[u] || lf > G.kset_hi[u]) continue;
+        const int fu = G.cF[u] + cart_sum(G.kset_lo[u], lf - 1) + cart_idx(lf, fx, fz);
+        const double gu = g * wk[u];
+        for (int r = 0; r < 3; ++r) {
+          if (G.bset_lo[r] < 0 || le < G.bset_lo[r] || le > G.bset_hi[r]) continue;
+          const int er = cart_sum(G.bset_lo[r], le - 1) + cart_idx(le, ex, ez);
+          X[G.oXs[r] + (int64_t)er * G.KF + fu] += gu * wb[r];
+        }
+      }
+    }
+  };
+  const int any = gen_walk(B, tk, s0, s1, s2, s3, kind, la, lb, lc, ld,
+                           la + lb + kind_bra_raise(kind), lc + ld + kind_ket_raise(kind), G.E,
+                           G.F, s_u, s_w, s_V, consume);
+  if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
+
+  if (!any) {
+    const int n = 9 * G.nAo * G.nBo * G.nCo * G.nDo;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int a = k % G.nAo; k /= G.nAo;
+      const int b = k % G.nBo; k /= G.nBo;
+      const int c = k % G.nCo; k /= G.nCo;
+      const int d = k % G.nDo;
+      const int comp = k / G.nDo;
+      out[comp * OS.comp + a * OS.s[0] + b * OS.s[1] + c * OS.s[2] + d * OS.s[3]] = 0;
+    }
+    return;
+  }
+
+  // pass 1: bra derivatives and transfer (Cartesian), H1[bc][a][b][kf] over
+  // every ket column kf; bc = 3p+q (ipip1) or p (ip1ip2)
+  const double AB[3] = {s0.r[0] - s1.r[0], s0.r[1] - s1.r[1], s0.r[2] - s1.r[2]};
+  {
+    const int n = G.nbc * G.nca * G.ncb * G.KF;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int kf = k % G.KF; k /= G.KF;
+      const int b = k % G.ncb; k /= G.ncb;
+      const int a = k % G.nca;
+      const int bc = k / G.nca;
+      int m = 0;
+      while ((m + 1) * (m + 2) / 2 <= a) ++m;
+      const int az = a - m * (m + 1) / 2;
+      int mb = 0;
+      while ((mb + 1) * (mb + 2) / 2 <= b) ++mb;
+      const int bz = b - mb * (mb + 1) / 2, bx = lb - mb, by = mb - bz;
+      const int av[3] = {la - m, m - az, az};
+      // transfer of set r at bra exponents av + dv
+      auto H = [&](int r, int dx, int dy, int dz) {
+        return hrr_bra(X + G.oXs[r], G.KF, kf, G.bset_lo[r], av[0] + dx, av[1] + dy, av[2] + dz,
+                       bx, by, bz, AB);
+      };
+      const int p = ipip ? bc / 3 : bc;
+      const int ep[3] = {p == 0, p == 1, p == 2};
+      const int ap = av[p];
+      double v;
+      if (!ipip) {
+        v = -H(1, ep[0], ep[1], ep[2]);
+        if (ap > 0) v += ap * H(0, -ep[0], -ep[1], -ep[2]);
+      } else {
+        const int q = bc - 3 * p;
+        const int eq[3] = {q == 0, q == 1, q == 2};
+        const int aq = av[q];
+        if (p == q) {
+          v = H(2, 2 * ep[0], 2 * ep[1], 2 * ep[2]) - (2 * ap + 1) * H(1, 0, 0, 0);
+          if (ap > 1) v += ap * (ap - 1) * H(0, -2 * ep[0], -2 * ep[1], -2 * ep[2]);
+        } else {
+          v = H(2, ep[0] + eq[0], ep[1] + eq[1], ep[2] + eq[2]);
+          if (aq > 0) v -= aq * H(1, ep[0] - eq[0], ep[1] - eq[1], ep[2] - eq[2]);
+          if (ap > 0) v -= ap * H(1, eq[0] - ep[0], eq[1] - ep[1], eq[2] - ep[2]);
+          if (ap > 0 && aq > 0)
+            v += ap * aq * H(0, -ep[0] - eq[0], -ep[1] - eq[1], -ep[2] - eq[2]);
+        }
+      }
+      X[G.oH1 + i] = v;
+    }
+  }
+  __syncthreads();
+  // pass 2: bra Cartesian -> spherical, H2[bc][ao][bo][kf]
+  {
+    const int n = G.nbc * G.nAo * G.nBo * G.KF;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int kf = k % G.KF; k /= G.KF;
+      const int bo = k % G.nBo; k /= G.nBo;
+      const int ao = k % G.nAo;
+      const int bc = k / G.nAo;
+      const double* h1 = X + G.oH1 + (int64_t)bc * G.nca * G.ncb * G.KF;
+      double v;
+      if (!A.sph) {
+        v = h1[((int64_t)ao * G.ncb + bo) * G.KF + kf];
+      } else {
+        const double* ca = c2s_coef + c2s_offset[la] + ao * G.nca;
+        const double* cb = c2s_coef + c2s_offset[lb] + bo * G.ncb;
+        v = 0;
+        for (int a = 0; a < G.nca; ++a) {
+          if (ca[a] == 0.0) continue;
+          double s = 0;
+          for (int b = 0; b < G.ncb; ++b)
+            if (cb[b] != 0.0) s += cb[b] * h1[((int64_t)a * G.ncb + b) * G.KF + kf];
+          v += ca[a] * s;
+        }
+      }
+      X[G.oH2 + i] = v;
+    }
+  }
+  __syncthreads();
+  // pass 3: ket derivative (ip1ip2) and transfer, H3[comp][ao][bo][c][d]
+  const double CD[3] = {s2.r[0] - s3.r[0], s2.r[1] - s3.r[1], s2.r[2] - s3.r[2]};
+  {
+    const int n = 9 * G.nAo * G.nBo * G.ncc * G.ncd;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int d = k % G.ncd; k /= G.ncd;
+      const int c = k % G.ncc; k /= G.ncc;
+      const int bo = k % G.nBo; k /= G.nBo;
+      const int ao = k % G.nAo;
+      const int comp = k / G.nAo;
+      int m = 0;
+      while ((m + 1) * (m + 2) / 2 <= c) ++m;
+      const int cz = c - m * (m + 1) / 2;
+      const int cv[3] = {lc - m, m - cz, cz};
+      int md = 0;
+      while ((md + 1) * (md + 2) / 2 <= d) ++md;
+      const int dz = d - md * (md + 1) / 2, dx = ld - md, dy = md - dz;
+      const int bc = ipip ? comp : comp / 3;
+      const double* h2 = X + G.oH2 + ((int64_t)(bc * G.nAo + ao) * G.nBo + bo) * G.KF;
+      // transfer of ket set u at ket exponents cv + dv
+      auto K = [&](int u, int ox, int oy, int oz) {
+        return hrr_bra(h2 + G.cF[u], 1, 0, G.kset_lo[u], cv[0] + ox, cv[1] + oy, cv[2] + oz, dx,
+                       dy, dz, CD);
+      };
+      double v;
+      if (ipip) {
+        v = K(0, 0, 0, 0);
+      } else {
+        const int q = comp - 3 * bc;
+        const int eq[3] = {q == 0, q == 1, q == 2};
+        v = -K(1, eq[0], eq[1], eq[2]);
+        if (cv[q] > 0) v += cv[q] * K(0, -eq[0], -eq[1], -eq[2]);
+      }
+      X[G.oH3 + i] = v;
+    }
+  }
+  __syncthreads();
+  // pass 4: ket Cartesian -> spherical and the store
+  {
+    const int n = 9 * G.nAo * G.nBo * G.nCo * G.nDo;
+    const int n_c = G.ncc * G.ncd;
+    for (int i = lane; i < n; i += GEN_LANES) {
+      int k = i;
+      const int ao = k % G.nAo; k /= G.nAo;
+      const int bo = k % G.nBo; k /= G.nBo;
+      const int co = k % G.nCo; k /= G.nCo;
+      const int dd = k % G.nDo;
+      const int comp = k / G.nDo;
+      const double* h3 = X + G.oH3 + ((int64_t)(comp * G.nAo + ao) * G.nBo + bo) * n_c;
+      double v;
+      if (!A.sph) {
+        v = h3[co * G.ncd + dd];
+      } else {
+        const double* cc = c2s_coef + c2s_offset[lc] + co * G.ncc;
+        const double* cd = c2s_coef + c2s_offset[ld] + dd * G.ncd;
+        v = 0;
+        for (int c = 0; c < G.ncc; ++c) {
+          if (cc[c] == 0.0) continue;
+          double s = 0;
+          for (int d = 0; d < G.ncd; ++d)
+            if (cd[d] != 0.0) s += cd[d] * h3[c * G.ncd + d];
+          v += cc[c] * s;
+        }
+      }
+      out[comp * OS.comp + ao * OS.s[0] + bo * OS.s[1] + co * OS.s[2] + dd * OS.s[3]] = v;
+    }
+  }
+}
+
 __global__ void rys_batch_kernel(int n, const double* __restrict__ x, int64_t count,
                                  double* __restrict__ u, double* __restrict__ w) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -636,7 +905,6 @@ __global__ void count_prims_kernel(BasisView B, BucketArgs A, unsigned long long
   const int64_t t = A.t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= A.t1) return;
   const Task tk = A.tasks[t];
-  const bool ip1 = kind_bra_der(A.kind);
   const bool three = kind_three(A.kind);
   const bool noscreen = A.kind == KIND_2C2E;
   const VShell s0 = B.vsh[tk.sh[0]], s1 = B.vsh[tk.sh[1]], s2 = B.vsh[tk.sh[2]];
@@ -644,8 +912,11 @@ __global__ void count_prims_kernel(BasisView B, BucketArgs A, unsigned long long
   const int nkl = three ? s2.nprim : s2.nprim * B.vsh[tk.sh[3]].nprim;
   const PairPrim* pij = B.pp + B.pair_off[(int64_t)tk.sh[0] * B.nv + tk.sh[1]];
   const PairPrim* pkl = three ? nullptr : B.pp + B.pair_off[(int64_t)tk.sh[2] * B.nv + tk.sh[3]];
-  const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]] * (A.la + A.lb + (ip1 ? 1 : 0));
-  const double ld_kl = three ? 0.0 : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]] * (A.lc + A.ld);
+  const double ld_ij = B.pair_ld[(int64_t)tk.sh[0] * B.nv + tk.sh[1]] *
+                       (A.la + A.lb + kind_bra_raise(A.kind));
+  const double ld_kl = three ? 0.0
+                             : B.pair_ld[(int64_t)tk.sh[2] * B.nv + tk.sh[3]] *
+                                   (A.lc + A.ld + kind_ket_raise(A.kind));
   const double E = three ? B.expcutoff_3c : B.expcutoff_4c;
   if (noscreen) {
     atomicAdd(counter, (unsigned long long)nij * nkl);
@@ -682,12 +953,17 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
   if (n <= 0) return hipSuccess;
   // grid.x is limited to 2^31-1 workgroups; callers split larger buckets
   const GenLayout G = gen_layout(A.kind, A.sph, A.la, A.lb, A.lc, A.ld);
-  const int nr = (A.la + A.lb + A.lc + (kind_three(A.kind) ? 0 : A.ld) +
-                  (kind_bra_der(A.kind) || kind_ket_der(A.kind) ? 1 : 0)) / 2 + 1;
+  const int nr = (A.la + A.lb + A.lc + (kind_three(A.kind) ? 0 : A.ld) + kind_bra_raise(A.kind) +
+                  kind_ket_raise(A.kind)) / 2 + 1;
   const int nrt = B.omega < 0 ? 2 * nr : nr;
   const size_t lds = sizeof(double) * 3 * nrt * (G.E + 1) * (G.F + 1);
-  if (lds > sizeof(double) * GEN_VCAP) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)n), dim3(GEN_LANES), lds, stream, B, A);
+  if (lds > sizeof(double) * GEN_VCAP || G.nE > GEN_ECAP || G.nF > GEN_FCAP)
+    return hipErrorInvalidValue;
+  if (A.scratch_stride < G.total) return hipErrorInvalidValue;
+  if (kind_hess(A.kind))
+    hipLaunchKernelGGL(eri_hess_kernel, dim3((unsigned)n), dim3(GEN_LANES), lds, stream, B, A);
+  else
+    hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)n), dim3(GEN_LANES), lds, stream, B, A);
   return hipGetLastError();
 }
 

@@ -12,9 +12,14 @@ namespace mcint {
 // LDS bound (doubles) of the generic kernel's recurrence arrays:
 // 3 directions x roots (doubled for short range) x (E+1)(F+1)
 constexpr int GEN_VCAP = 3 * 2 * NRYS_MAX * 90;
+// component-table bounds of the generic kernels' union e-list and f-list:
+// all Cartesian components of l <= 14 (ipip1 of (66|ss) reaches e = 14)
+constexpr int GEN_ECAP = 680;
+constexpr int GEN_FCAP = 680;
 
 struct GenLayout {
-  int e_lo, E, F;         // e shells e_lo..E, f shells lc..F
+  int e_lo, E, F;         // e shells e_lo..E, f shells f_lo..F
+  int f_lo;
   int nE, nF;             // union e-list and f-list lengths
   int p_lo, p_hi;         // shells of the main (or 2a-weighted) set
   int m_lo, m_hi;         // shells of the lowered set (m_hi < m_lo: empty)
@@ -22,7 +27,34 @@ struct GenLayout {
   int ncomp;
   int nca, ncb, nAo, nBo, ncc, ncd, nCo, nDo;
   int64_t oX0, oXM, oH1, oH2, oH3, total;
+  // second derivatives (int2e_ipip1, int2e_ip1ip2): weighted row sets of the
+  // bra (role 0: lowered, weight 1; 1: weight 2a_i; 2: weight 4a_i^2) and the
+  // ket (role 0: lowered, weight 1; 1: raised, weight 2a_k), -1 when absent;
+  // set r covers shells lo..hi, X holds set r's rows (NEs) x all ket columns
+  // (KF = sum of NFs, set u's columns from cF[u]); nbc bra components
+  int bset_lo[3], bset_hi[3], NEs[3];
+  int64_t oXs[3];
+  int kset_lo[2], kset_hi[2], NFs[2], cF[2];
+  int KF, nbc;
 };
+
+// shell range of bra role r / ket role u of a second-derivative class
+__host__ __device__ inline void hess_sets(int kind, int la, int lb, int lc, int ld, int* blo,
+                                          int* bhi, int* klo, int* khi) {
+  for (int r = 0; r < 3; ++r) blo[r] = bhi[r] = -1;
+  for (int u = 0; u < 2; ++u) klo[u] = khi[u] = -1;
+  if (kind == KIND_2E_IPIP1) {
+    if (la >= 2) { blo[0] = la - 2; bhi[0] = la + lb - 2; }
+    blo[1] = la; bhi[1] = la + lb;
+    blo[2] = la + 2; bhi[2] = la + lb + 2;
+    klo[0] = lc; khi[0] = lc + ld;  // one plain ket set, stored in role 0
+  } else {
+    if (la >= 1) { blo[0] = la - 1; bhi[0] = la + lb - 1; }
+    blo[1] = la + 1; bhi[1] = la + lb + 1;
+    if (lc >= 1) { klo[0] = lc - 1; khi[0] = lc + ld - 1; }
+    klo[1] = lc + 1; khi[1] = lc + ld + 1;
+  }
+}
 
 __host__ __device__ inline int cart_sum(int lo, int hi) {
   int s = 0;
@@ -32,6 +64,56 @@ __host__ __device__ inline int cart_sum(int lo, int hi) {
 
 __host__ __device__ inline GenLayout gen_layout(int kind, int sph, int la, int lb, int lc, int ld) {
   GenLayout g;
+  if (kind_hess(kind)) {
+    int blo[3], bhi[3], klo[2], khi[2];
+    hess_sets(kind, la, lb, lc, ld, blo, bhi, klo, khi);
+    g.ncomp = 9;
+    g.nbc = kind == KIND_2E_IPIP1 ? 9 : 3;
+    g.e_lo = 99; g.E = 0; g.f_lo = 99; g.F = 0;
+    g.KF = 0;
+    for (int u = 0; u < 2; ++u) {
+      g.kset_lo[u] = klo[u];
+      g.kset_hi[u] = khi[u];
+      g.NFs[u] = klo[u] >= 0 ? cart_sum(klo[u], khi[u]) : 0;
+      g.cF[u] = g.KF;
+      g.KF += g.NFs[u];
+      if (klo[u] >= 0) {
+        g.f_lo = klo[u] < g.f_lo ? klo[u] : g.f_lo;
+        g.F = khi[u] > g.F ? khi[u] : g.F;
+      }
+    }
+    int64_t o = 0;
+    for (int r = 0; r < 3; ++r) {
+      g.bset_lo[r] = blo[r];
+      g.bset_hi[r] = bhi[r];
+      g.NEs[r] = blo[r] >= 0 ? cart_sum(blo[r], bhi[r]) : 0;
+      g.oXs[r] = o;
+      o += (int64_t)g.NEs[r] * g.KF;
+      if (blo[r] >= 0) {
+        g.e_lo = blo[r] < g.e_lo ? blo[r] : g.e_lo;
+        g.E = bhi[r] > g.E ? bhi[r] : g.E;
+      }
+    }
+    g.nE = cart_sum(g.e_lo, g.E);
+    g.nF = cart_sum(g.f_lo, g.F);
+    g.p_lo = g.p_hi = g.m_lo = g.m_hi = g.NE0 = g.NEM = 0;
+    g.nca = ncart(la);
+    g.ncb = ncart(lb);
+    g.ncc = ncart(lc);
+    g.ncd = ncart(ld);
+    g.nAo = sph ? nsph(la) : g.nca;
+    g.nBo = sph ? nsph(lb) : g.ncb;
+    g.nCo = sph ? nsph(lc) : g.ncc;
+    g.nDo = sph ? nsph(ld) : g.ncd;
+    g.oX0 = g.oXM = 0;
+    g.oH1 = o;
+    g.oH2 = g.oH1 + (int64_t)g.nbc * g.nca * g.ncb * g.KF;
+    g.oH3 = g.oH2 + (int64_t)g.nbc * g.nAo * g.nBo * g.KF;
+    g.total = g.oH3 + (int64_t)9 * g.nAo * g.nBo * g.ncc * g.ncd;
+    return g;
+  }
+  g.f_lo = lc;
+  g.nbc = g.KF = 0;
   const bool ip1 = kind_bra_der(kind);
   const bool ip2 = kind_ket_der(kind);
   const bool three = kind_three(kind);

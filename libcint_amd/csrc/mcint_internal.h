@@ -27,14 +27,28 @@ enum IntorKind : int {
   KIND_3C2E_IP1 = 3,  // derivative on shell i (int3c2e_ip1)
   KIND_3C2E_IP2 = 4,  // derivative on the auxiliary shell k (int3c2e_ip2)
   KIND_2C2E = 5,      // (i|k) (int2c2e)
+  KIND_2E_IPIP1 = 6,  // second derivative on shell i (int2e_ipip1)
+  KIND_2E_IP1IP2 = 7, // derivatives on shells i and k (int2e_ip1ip2)
 };
-__host__ __device__ constexpr bool kind_three(int k) { return k >= KIND_3C2E; }
+constexpr int NKIND = 8;
+__host__ __device__ constexpr bool kind_three(int k) { return k >= KIND_3C2E && k <= KIND_2C2E; }
 __host__ __device__ constexpr bool kind_bra_der(int k) {
   return k == KIND_2E_IP1 || k == KIND_3C2E_IP1;
 }
 __host__ __device__ constexpr bool kind_ket_der(int k) { return k == KIND_3C2E_IP2; }
+__host__ __device__ constexpr bool kind_hess(int k) {
+  return k == KIND_2E_IPIP1 || k == KIND_2E_IP1IP2;
+}
 __host__ __device__ constexpr int kind_ncomp(int k) {
-  return kind_bra_der(k) || kind_ket_der(k) ? 3 : 1;
+  return kind_hess(k) ? 9 : (kind_bra_der(k) || kind_ket_der(k) ? 3 : 1);
+}
+// angular raises of the bra and the ket pair (the recurrence's extra steps,
+// the Rys order and the screening's distance terms)
+__host__ __device__ constexpr int kind_bra_raise(int k) {
+  return k == KIND_2E_IPIP1 ? 2 : (kind_bra_der(k) || k == KIND_2E_IP1IP2 ? 1 : 0);
+}
+__host__ __device__ constexpr int kind_ket_raise(int k) {
+  return kind_ket_der(k) || k == KIND_2E_IP1IP2 ? 1 : 0;
 }
 // shells of the caller's tuple: 4, 3 (ij|k) or 2 (i|k)
 __host__ __device__ constexpr int kind_axes(int k) {

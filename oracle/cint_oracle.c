@@ -853,7 +853,7 @@ static void oc_nabla_k(double *f, const double *g, const OcEnv *e, double ak)
 }
 
 /* Nine components of a second derivative, src/autocode/hess.c:1125-1174
- * (int2e_ipip1) and :1290-1338 (int2e_ip1ip2): component 3p+q carries the
+ * (int2e_ipip1) and :1289-1338 (int2e_ip1ip2): component 3p+q carries the
  * first operator along axis p and the second along q; fa is the first
  * operator applied to g, fb the second applied to g, fab the first applied
  * to fb (axis p = q).  gout[9n + 3p + q]. */

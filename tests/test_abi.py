@@ -34,7 +34,10 @@ def test_header_declares_reference_entry_points():
               "CINTdel_2e_optimizer", "cint2e_sph", "cint2e_cart", "cint2e_sph_optimizer",
               "cint2e_ip1_sph", "CINTgto_norm", "CINTcgto_spheric", "CINTcgto_cart",
               "CINTtot_cgto_spheric", "CINTshells_spheric_offset", "CINTrys_roots",
-              "int2e_sph_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch"):
+              "int2e_sph_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch", "int2e_ip2_sph",
+              "int2e_ip2_optimizer", "int2e_ipip1_sph", "int2e_ipip1_cart", "int2e_ipip1_optimizer",
+              "int2e_ip1ip2_sph", "int2e_ip1ip2_cart", "int2e_ip1ip2_optimizer", "int3c2e_ip1_sph",
+              "int3c2e_ip2_sph", "int2c2e_sph"):
         assert n in names, n
 
 

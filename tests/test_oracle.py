@@ -495,7 +495,7 @@ def test_second_derivatives_finite_difference(orc, name, first, slot):
     component p of int2e_ip1 (shell i moved), and of int2e_ip1ip2 =
     (d/dr_p i j|d/dr_q k l) it is -d/dC_q of the same (shell k moved); central
     differences (h = 1e-4) of the ip1 integrals, which the cint2e_ip1_sph
-    checksum pins (src/autocode/hess.c:1125-1174, 1290-1338)."""
+    checksum pins (src/autocode/hess.c:1125-1174, 1289-1338)."""
     from libcint_amd.synthetic import make_molecule
     m = make_molecule("c3")
     rng = np.random.default_rng(41 + slot + (name.endswith("sph")))
