@@ -182,6 +182,11 @@ def gradient_section(reps: int = 3) -> dict:
     dmt = torch.as_tensor(dm, device="cuda")
     e64 = float(jk.energy(dmt, vj, vk))
     jb.close()
+    jbs = jk.JKBuilder(B, fused=False)
+    t_jk_stored, (vjs, vks) = timed(lambda: jbs.run(dm))
+    fused_vs_stored = max(float((vjs - vj).abs().max()), float((vks - vk).abs().max()))
+    jbs.close()
+    del vjs, vks
     jb32 = jk.JKBuilder(B, fp32_threshold=1e-7, dmax=float(np.abs(dm).max()))
     fp32_tasks = jb32.stats()["fp32_tasks"]
     t_jk32, (vj32, vk32) = timed(lambda: jb32.run(dm))
@@ -238,6 +243,8 @@ def gradient_section(reps: int = 3) -> dict:
         "ip1_gradient_mixed_max_abs_error": g32_err,
         "gradient_translational_residual": gsum,
         "jk_fp64_ms": t_jk * 1e3,
+        "jk_fp64_stored_buffer_ms": t_jk_stored * 1e3,
+        "jk_fused_vs_stored_max_abs_diff": fused_vs_stored,
         "jk_mixed_ms": t_jk32 * 1e3,
         "jk_mixed_fp32_quartets": int(fp32_tasks),
         "jk_mixed_max_abs_dJ": dj,
@@ -260,6 +267,30 @@ def gradient_section(reps: int = 3) -> dict:
                           "the FP32 tier for Schwarz x max|D|^2 < 1e-7",
         },
     }
+
+
+def fused_jk_section(plan, m, reps: int, t_step: float) -> dict:
+    """J and K of the timed workload (the same plan over the same unique
+    quartets) with the density contraction inside the integral kernels: no
+    ERI buffer; against the time of the ERI step that stores every block."""
+    import torch
+    from libcint_amd.synthetic import random_density
+    nao = m.nao_sph()
+    dm = torch.as_tensor(random_density(nao, seed=13), device="cuda")
+    vj = torch.empty((nao, nao), dtype=torch.float64, device="cuda")
+    vk = torch.empty_like(vj)
+    plan.execute_jk(dm.data_ptr(), nao, vj.data_ptr(), vk.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.execute_jk(dm.data_ptr(), nao, vj.data_ptr(), vk.data_ptr())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    e = float(0.5 * (dm * vj).sum() - 0.25 * (dm * vk).sum())
+    return {"workload": f"{m.name}: J and K for a random density D = A A^T / N, {nao} AOs",
+            "ms": dt * 1e3, "eri_step_ms": t_step * 1e3, "vs_eri_step": dt / t_step,
+            "eri_buffer_bytes": 0, "energy": e,
+            "device_memory_beyond_plan_bytes": 2 * nao * nao * 8}
 
 
 def c2_section(reps: int = 2) -> dict:
@@ -569,6 +600,11 @@ def main():
     result["sampled_check"] = sampled_check(db, B, plan, quart, out)
     log(f"sampled_check: {result['sampled_check']}")
     del out
+    torch.cuda.empty_cache()
+    if not args.no_gradient:
+        log("J/K fused into the integral kernels on the same plan (no ERI buffer) ...")
+        result["fused_jk"] = fused_jk_section(plan, m, args.steps, t_step=dt / args.steps)
+        log(f"fused_jk: {result['fused_jk']}")
     plan.close()
     if not args.no_gradient:
         log("gradient / J/K legs ...")

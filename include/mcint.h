@@ -346,6 +346,20 @@ int mcint_plan_count_primitives(mcint_plan *p, int64_t *per_bucket);
 int mcint_plan_execute_grad(mcint_plan *p, const double *dm_dev, int nao, double *grad_dev,
                             void *hip_stream);
 
+/* Coulomb and exchange matrices fused into the integral kernels: executes
+ * an int2e_sph plan over unique quartets (i>=j, k>=l, ij>=kl, each once)
+ * and, instead of storing the blocks, contracts each block with the
+ * density as it is formed:
+ *   vj[p,q] = sum_rs (pq|rs) D[r,s],   vk[p,r] = sum_qs (pq|rs) D[q,s]
+ * (overwritten; either nullable).  dm_dev: nao x nao spherical density
+ * (row-major, symmetric).  No ERI buffer: device memory is the plan's
+ * scratch and two nao x nao matrices.  Plain and long-range operators;
+ * mixed-precision plans digest their FP32 blocks too.  Asynchronous on
+ * hip_stream.  (The integral-side layer libcint leaves to its callers,
+ * PySCF's _vhf.) */
+int mcint_plan_execute_jk(mcint_plan *p, const double *dm_dev, int nao, double *vj_dev,
+                          double *vk_dev, void *hip_stream);
+
 /* Coulomb / exchange matrices from computed int2e_sph blocks: for every
  * quartet q of shls_dev (4 ints each, device) with its block at
  * eri_dev + off_dev[q], and every distinct permutation (pq|rs) of it,

@@ -107,6 +107,8 @@ def lib():
     L.mcint_plan_count_primitives.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.mcint_plan_execute_grad.restype = ctypes.c_int
     L.mcint_plan_execute_grad.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    L.mcint_plan_execute_jk.restype = ctypes.c_int
+    L.mcint_plan_execute_jk.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp]
     L.mcint_jk_digest.restype = ctypes.c_int
     L.mcint_jk_digest.argtypes = [vp, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp]
     L.mcint_jk_plan_create.restype = ctypes.c_int
@@ -369,6 +371,17 @@ class Plan:
                                              ctypes.c_void_p(grad_ptr),
                                              ctypes.c_void_p(stream_ptr or None)),
                "mcint_plan_execute_grad")
+
+    def execute_jk(self, dm_ptr: int, nao: int, vj_ptr: int = 0, vk_ptr: int = 0,
+                   stream_ptr: int = 0):
+        """J and K (nao x nao, device, overwritten; either pointer 0 to skip)
+        contracted inside the integral kernels of an int2e_sph plan over
+        unique quartets; no ERI buffer."""
+        _check(lib().mcint_plan_execute_jk(self.handle, ctypes.c_void_p(dm_ptr), int(nao),
+                                           ctypes.c_void_p(vj_ptr or None),
+                                           ctypes.c_void_p(vk_ptr or None),
+                                           ctypes.c_void_p(stream_ptr or None)),
+               "mcint_plan_execute_jk")
 
     def close(self):
         if self.handle:

@@ -524,6 +524,8 @@ struct mcint_plan {
   int* flags_d = nullptr;
   int64_t flags_size = 0;
   double* gslots_d = nullptr;  // gradient digest accumulators
+  double* jk_d = nullptr;      // J', K' half matrices of the fused J/K digest
+  int64_t jk_n = 0;            // nao * nao they hold
   int64_t stats[6] = {0, 0, 0, 0, 0, 0};
   int64_t stats_fp32 = 0;
   // per-branch scratch: scratch_d / flags_d hold scratch_nbr regions of
@@ -556,6 +558,7 @@ static void plan_free(mcint_plan* p) {
   dfree(p->scratch_d);
   dfree(p->flags_d);
   dfree(p->gslots_d);
+  dfree(p->jk_d);
   delete p;
 }
 
@@ -1033,6 +1036,8 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.nao = 0;
   A.nslot = 1;
   A.grad_stride = 0;
+  A.jk_j = nullptr;
+  A.jk_k = nullptr;
   return A;
 }
 
@@ -1238,6 +1243,74 @@ extern "C" int mcint_plan_execute_grad(mcint_plan* p, const double* dm_dev, int 
       if (const int st = launch(bk, stream, plan_scratch(p))) return st;
   }
   HIPCHK(launch_grad_reduce(p->gslots_d, GRAD_SLOTS, n, grad_dev, stream));
+  return MCINT_OK;
+}
+
+extern "C" int mcint_plan_execute_jk(mcint_plan* p, const double* dm_dev, int nao,
+                                     double* vj_dev, double* vk_dev, void* hip_stream) {
+  if (!p || !dm_dev || (!vj_dev && !vk_dev) || nao <= 0) {
+    set_err("mcint_plan_execute_jk: null argument");
+    return MCINT_ERR_ARG;
+  }
+  if (p->intor != MCINT_INT2E_SPH) {
+    set_err("mcint_plan_execute_jk: the plan must be int2e_sph over unique quartets");
+    return MCINT_ERR_ARG;
+  }
+  if (p->b->omega < 0) {
+    set_err("mcint_plan_execute_jk: the short-range operator has no fused digest; "
+            "execute the plan and digest its buffer (mcint_jk_plan_execute)");
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  const int nao_b = CINTtot_cgto_spheric(p->b->bas.data(), p->b->nbas);
+  if (nao != nao_b) {
+    set_err("mcint_plan_execute_jk: density dimension does not match the basis");
+    return MCINT_ERR_ARG;
+  }
+  hipStream_t stream = (hipStream_t)hip_stream;
+  const int nbr = std::max(1, std::min<int>(g_graph_branches, (int)p->buckets.size()));
+  if (const int st = ensure_scratch(p, nbr)) return st;
+  const int64_t nn = (int64_t)nao * nao;
+  if (p->jk_n < nn) {
+    if (p->jk_d) HIPCHK(hipDeviceSynchronize());
+    dfree(p->jk_d);
+    HIPCHK(dalloc(&p->jk_d, (size_t)(2 * nn)));
+    p->jk_n = nn;
+  }
+  double* Jp = vj_dev ? p->jk_d : nullptr;
+  double* Kp = vk_dev ? p->jk_d + nn : nullptr;
+  HIPCHK(launch_zero(p->jk_d, 2 * nn, stream));
+  const BasisView B = plan_view(p);
+  auto launch = [&](const Bucket& bk, hipStream_t s, const ScratchView& S) -> int {
+    BucketArgs A = bucket_args(p, bk, nullptr, nullptr);
+    A.dm = dm_dev;
+    A.nao = nao;
+    A.jk_j = Jp;
+    A.jk_k = Kp;
+    A.scratch = S.blocks;
+    A.flags = S.flags;
+    if (bk.mode == 1) {
+      HIPCHK(launch_fast(B, A, s));
+      return MCINT_OK;
+    }
+    const int64_t chunk = std::max<int64_t>(1, slice_tasks(bk, S));
+    for (int64_t t = bk.t0; t < bk.t1; t += chunk) {
+      A.t0 = t;
+      A.t1 = std::min(bk.t1, t + chunk);
+      if (bk.mode == 2)
+        HIPCHK(launch_chunked(B, A, s));
+      else
+        HIPCHK(launch_generic(B, A, s));
+    }
+    return MCINT_OK;
+  };
+  if (nbr > 1) {
+    if (const int st = launch_branched(p, stream, nbr, launch)) return st;
+  } else {
+    for (const Bucket& bk : p->buckets)
+      if (const int st = launch(bk, stream, plan_scratch(p))) return st;
+  }
+  if (vj_dev) HIPCHK(launch_symmetrize(Jp, vj_dev, nao, stream));
+  if (vk_dev) HIPCHK(launch_symmetrize(Kp, vk_dev, nao, stream));
   return MCINT_OK;
 }
 
@@ -1966,6 +2039,8 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
   A.dm = nullptr;
   A.grad = nullptr;
   A.nslot = 1;
+  A.jk_j = nullptr;
+  A.jk_k = nullptr;
   e = hipMemcpyAsync(c->task_d, c->task_h, sizeof(Task) * nt, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = mode == 1 ? launch_fast(B, A, c->stream)

@@ -21,6 +21,21 @@ __device__ __forceinline__ int cart_idx(int l, int lx, int lz) {
   return m * (m + 1) / 2 + lz;
 }
 
+// J/K digest of unique quartets.  With the degeneracy factor
+// f = (i==j ? 1/2 : 1) (k==l ? 1/2 : 1) (ij==kl ? 1/2 : 1) of a quartet of
+// real shells (i>=j, k>=l, ij>=kl in any order), the sum over its 8 index
+// maps reduces to six block updates of half matrices
+//     J'_ij += 2f (ij|kl) D_kl      J'_kl += 2f (ij|kl) D_ij
+//     K'_ik += f (ij|kl) D_jl       K'_il += f (ij|kl) D_jk
+//     K'_jk += f (ij|kl) D_il       K'_jl += f (ij|kl) D_ik
+// and J = J' + J'^T, K = K' + K'^T.  The set of updates is the same for
+// every order of the four shells within the symmetry group, so kernels
+// may digest a quartet in their canonical shell order.
+__host__ __device__ inline double jk_degeneracy(int i, int j, int k, int l) {
+  return (i == j ? 0.5 : 1.0) * (k == l ? 0.5 : 1.0) *
+         (((i == k && j == l) || (i == l && j == k)) ? 0.5 : 1.0);
+}
+
 // Gradient digest: adds the per-team partials g[k][x] (atom[k] < 0: none)
 // of one block to slot (block % nslot) of the accumulators, one atomic per
 // run of equal atoms and per component.  Call after a barrier that made g

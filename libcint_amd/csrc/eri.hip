@@ -575,7 +575,12 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
   {
     const int nD = three ? 1 : G.nDo;
     const int n = G.ncomp * G.nAo * G.nBo * G.nCo * nD;
-    const bool digest = A.dm != nullptr;
+    // J/K digest of a plain four-centre block (six block updates of
+    // jk_degeneracy, device_common.h; one atomic per element and update:
+    // the generic path carries few plain classes), or the gradient digest
+    const bool jk = A.jk_j != nullptr || A.jk_k != nullptr;
+    const bool digest = A.dm != nullptr && !jk;
+    const double fjk = jk ? jk_degeneracy(s0.shell, s1.shell, s2.shell, s3.shell) : 0.0;
     double gpart[3] = {0.0, 0.0, 0.0};
     const double fkl = (!three && s2.shell != s3.shell) ? 2.0 : 1.0;
     for (int i = lane; i < n; i += GEN_LANES) {
@@ -607,7 +612,21 @@ eri_generic_kernel(BasisView B, BucketArgs A) {
           v += cc[c] * s;
         }
       }
-      if (digest) {
+      if (jk) {
+        const int64_t N = A.nao;
+        const int64_t P = s0.ao + ao, Q = s1.ao + bo, R = s2.ao + co, S = s3.ao + dd;
+        const double* D = A.dm;
+        if (A.jk_j) {
+          atomicAdd(A.jk_j + P * N + Q, 2.0 * fjk * v * D[R * N + S]);
+          atomicAdd(A.jk_j + R * N + S, 2.0 * fjk * v * D[P * N + Q]);
+        }
+        if (A.jk_k) {
+          atomicAdd(A.jk_k + P * N + R, fjk * v * D[Q * N + S]);
+          atomicAdd(A.jk_k + P * N + S, fjk * v * D[Q * N + R]);
+          atomicAdd(A.jk_k + Q * N + R, fjk * v * D[P * N + S]);
+          atomicAdd(A.jk_k + Q * N + S, fjk * v * D[P * N + R]);
+        }
+      } else if (digest) {
         // dE/dA = sum (d/dr i j|kl) [-2 D_ij D_kl + (D_ik D_jl + D_il D_jk)/2], times 2
         // when k and l are different shells (the list holds k >= l only)
         const int64_t N = A.nao;

@@ -498,7 +498,12 @@ constexpr int min_waves(int la, int lb, int lc, int ld, int der, int chunk) {
   }
 }
 
-template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0, bool SPH = true>
+// JK: instead of the store, the J/K digest of the block (jk_degeneracy in
+// device_common.h): each lane sums the products of its elements with the
+// density into the six block updates, the team adds its lanes' sums, and
+// the lanes share the atomic additions to J', K'.
+template <int LA, int LB, int LC, int LD, typename R = double, int DER = 0, bool SPH = true,
+          bool JK = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(min_waves(LA, LB, LC, LD, DER, -1))))
 eri_fast_kernel(BasisView Bv, BucketArgs A) {
@@ -542,7 +547,9 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
 
   constexpr int NA = C::NA, NB = C::NB, NC = C::NC, ND = C::ND;
   constexpr int NAo = nout(LA, SPH), NBo = nout(LB, SPH), NCo = nout(LC, SPH), NDo = nout(LD, SPH);
-  if (!any) {
+  // (the J/K digest keeps screened tasks in the wave: their accumulators
+  // are zero and they take part in the wave's reduction)
+  if (!JK && !any) {
     sfor<NAo * NBo * NCo * NDo>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr int ao = k / (NBo * NCo * NDo), bo = (k / (NCo * NDo)) % NBo;
@@ -595,6 +602,115 @@ eri_fast_kernel(BasisView Bv, BucketArgs A) {
       Ys[(ao * NBo + bo) * NF + j] = s;
     });
   });
+  if constexpr (JK) {
+    static_assert(SPH && (DER == 0), "the J/K digest takes plain spherical blocks");
+    constexpr int O1 = NAo * NBo, O2 = O1 + NCo * NDo, O3 = O2 + NAo * NCo;
+    constexpr int O4 = O3 + NAo * NDo, O5 = O4 + NBo * NCo, M = O5 + NBo * NDo;
+    double g[M];
+    sfor<M>([&](auto I) { g[decltype(I)::value] = 0.0; });
+    const int64_t N = A.nao;
+    const double* __restrict__ D = A.dm;
+    const int64_t P0 = sa.ao, Q0 = sb.ao, R0 = sc.ao, S0 = sd.ao;
+    sfor<NAo * NBo>([&](auto RW) {
+      constexpr int row = decltype(RW)::value;
+      constexpr int ao = row / NBo, bo = row % NBo;
+      R Zc[NC * ND];
+      sfor<NC * ND>([&](auto CD) {
+        constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+        constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
+        constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
+        R s = R(0);
+        sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
+          constexpr int tt = decltype(TT)::value;
+          constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
+          constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
+          constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
+          const R pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+          s = fma(R(cb) * pw, Ys[row * NF + f], s);
+        });
+        Zc[c * ND + d] = s;
+      });
+      const double* __restrict__ DP = D + (P0 + ao) * N;
+      const double* __restrict__ DQ = D + (Q0 + bo) * N;
+      const double dpq = DP[Q0 + bo];
+      sfor<NCo * NDo>([&](auto OD) {
+        constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+        constexpr int k = (row * NCo + co) * NDo + dd;
+        if ((k & (T - 1)) == tl) {
+          R s = R(0);
+          sfor<NC * ND>([&](auto CD) {
+            constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+            constexpr double cc = c2x(SPH, LC, co, c) * c2x(SPH, LD, dd, d);
+            if constexpr (cc != 0.0) s = fma(R(cc), Zc[c * ND + d], s);
+          });
+          const double v = (double)s;
+          const int64_t Rr = R0 + co, Ss = S0 + dd;
+          g[row] = fma(v, D[Rr * N + Ss], g[row]);
+          g[O1 + co * NDo + dd] = fma(v, dpq, g[O1 + co * NDo + dd]);
+          g[O2 + ao * NCo + co] = fma(v, DQ[Ss], g[O2 + ao * NCo + co]);
+          g[O3 + ao * NDo + dd] = fma(v, DQ[Rr], g[O3 + ao * NDo + dd]);
+          g[O4 + bo * NCo + co] = fma(v, DP[Ss], g[O4 + bo * NCo + co]);
+          g[O5 + bo * NDo + dd] = fma(v, DP[Rr], g[O5 + bo * NDo + dd]);
+        }
+      });
+    });
+    for (int off = 1; off < T; off <<= 1)
+      sfor<M>([&](auto I) {
+        constexpr int m = decltype(I)::value;
+        g[m] += __shfl_xor(g[m], off);
+      });
+    const double f = jk_degeneracy(sa.shell, sb.shell, sc.shell, sd.shell);
+    sfor<M>([&](auto I) {
+      constexpr int m = decltype(I)::value;
+      g[m] *= m < O2 ? 2.0 * f : f;
+    });
+    // block r of the six: rows from rs[r], columns from cs[r] of J' or K'
+    const int64_t rs[6] = {P0, R0, P0, P0, Q0, Q0};
+    const int64_t cs[6] = {Q0, S0, R0, S0, R0, S0};
+    // Teams of one wave that update the same block sum their values first
+    // (the atomics are the digest's cost): per block, runs of consecutive
+    // teams with equal block origin form segments, a segmented suffix sum
+    // over the teams leaves each segment's total in its first team, and
+    // only first teams add.  Needs every lane of the wave present (the
+    // bucket's last wave adds team by team).
+    bool head[6] = {true, true, true, true, true, true};
+    if (T < 64 && __ballot(1) == ~0ull) {
+      const int lane = threadIdx.x & 63;
+      sfor<6>([&](auto RR) {
+        constexpr int r = decltype(RR)::value;
+        constexpr int o0 = r == 0 ? 0 : r == 1 ? O1 : r == 2 ? O2 : r == 3 ? O3 : r == 4 ? O4 : O5;
+        constexpr int o1 = r == 0 ? O1 : r == 1 ? O2 : r == 2 ? O3 : r == 3 ? O4 : r == 4 ? O5 : M;
+        const int key = (int)(rs[r] * N + cs[r]);
+        const int kprev = __shfl_up(key, T);
+        head[r] = lane < T || kprev != key;
+        // segment number of this lane's team: first teams up to it
+        const unsigned long long firsts = __ballot(head[r] && tl == 0);
+        const int seg = __popcll(lane == 63 ? firsts : firsts & ((2ull << lane) - 1));
+        for (int sh = T; sh < 64; sh <<= 1) {
+          const int sd_ = __shfl_down(seg, sh);
+          const bool take = lane + sh < 64 && sd_ == seg;
+          sfor<o1 - o0>([&](auto I) {
+            constexpr int m = o0 + decltype(I)::value;
+            const double vd = __shfl_down(g[m], sh);
+            if (take) g[m] += vd;
+          });
+        }
+      });
+    }
+    double* __restrict__ Jp = A.jk_j;
+    double* __restrict__ Kp = A.jk_k;
+    sfor<M>([&](auto I) {
+      constexpr int m = decltype(I)::value;
+      constexpr int r = m < O1 ? 0 : m < O2 ? 1 : m < O3 ? 2 : m < O4 ? 3 : m < O5 ? 4 : 5;
+      constexpr int o0 = r == 0 ? 0 : r == 1 ? O1 : r == 2 ? O2 : r == 3 ? O3 : r == 4 ? O4 : O5;
+      constexpr int ncol = r == 0 ? NBo : (r == 1 || r == 3 || r == 5) ? NDo : NCo;
+      constexpr int i = (m - o0) / ncol, j = (m - o0) % ncol;
+      double* __restrict__ dst = r < 2 ? Jp : Kp;
+      if (head[r] && (m & (T - 1)) == tl && dst && g[m] != 0.0)
+        atomicAdd(dst + (rs[r] + i) * N + cs[r] + j, g[m]);
+    });
+    return;
+  }
   // ket, row by row, then the store (lane tl writes every T-th element)
   sfor<NAo * NBo>([&](auto RW) {
     constexpr int row = decltype(RW)::value;
@@ -927,6 +1043,178 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
   if (valid) epi_ket_store<LA, LB, LC, LD, SPH>(Bv, tk, Ys, A.out, any, tl, TE);
 }
 
+// J/K digest variant of the chunked epilogue (plain spherical blocks): the
+// bra pass as in eri_epi_kernel; then the lane that owns a row (ao, bo)
+// forms its elements and adds their products with the density into LDS
+// accumulators of the task (ds_add): the six block updates of
+// jk_degeneracy (device_common.h), J'_ab, J'_cd, K'_ac, K'_ad, K'_bc,
+// K'_bd.  Consecutive tasks of the workgroup that update the same block
+// are summed in LDS before the one atomic per entry.
+template <int LA, int LB, int LC, int LD>
+struct EpiJkCfg {
+  using P = EpiCfg<LA, LB, LC, LD, true>;
+  static constexpr int NAo = P::NAo, NBo = P::NBo, NCo = P::NCo, NDo = P::NDo;
+  static constexpr int O1 = NAo * NBo, O2 = O1 + NCo * NDo, O3 = O2 + NAo * NCo;
+  static constexpr int O4 = O3 + NAo * NDo, O5 = O4 + NBo * NCo, MJK = O5 + NBo * NDo;
+  static constexpr int PER = P::PER + MJK;
+  static constexpr int LDS_MAX = 7680;  // doubles (60 KB; the rest: keys and weights)
+  static constexpr int TPB = P::TPB * PER <= LDS_MAX ? P::TPB : (LDS_MAX / PER > 0 ? LDS_MAX / PER : 1);
+  static constexpr int THREADS = TPB * P::TE;
+  static_assert(PER <= LDS_MAX, "J/K epilogue exceeds the LDS of a workgroup");
+};
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void epi_ket_jk(const BasisView& Bv, const Task& tk,
+                                           const double* __restrict__ Ys, double* G,
+                                           const BucketArgs& A, int r0, int rs) {
+  using C = FastClass<LA, LB, LC, LD>;
+  using Q = EpiJkCfg<LA, LB, LC, LD>;
+  constexpr int NF = C::NF, NC = C::NC, ND = C::ND;
+  constexpr int NBo = Q::NBo, NCo = Q::NCo, NDo = Q::NDo;
+  const VShell& sa = Bv.vsh[tk.sh[0]];
+  const VShell& sb = Bv.vsh[tk.sh[1]];
+  const VShell& sc = Bv.vsh[tk.sh[2]];
+  const VShell& sd = Bv.vsh[tk.sh[3]];
+  const double cd[3] = {sc.r[0] - sd.r[0], sc.r[1] - sd.r[1], sc.r[2] - sd.r[2]};
+  double cdp[3][LD + 1];
+  sfor<3>([&](auto DI) {
+    constexpr int d = decltype(DI)::value;
+    cdp[d][0] = 1.0;
+    sfor<LD>([&](auto K) { cdp[d][decltype(K)::value + 1] = cdp[d][decltype(K)::value] * cd[d]; });
+  });
+  const int64_t N = A.nao;
+  const double* __restrict__ D = A.dm;
+  const int64_t P0 = sa.ao, Q0 = sb.ao, R0 = sc.ao, S0 = sd.ao;
+  for (int row = r0; row < Q::NAo * NBo; row += rs) {
+    const int ao = row / NBo, bo = row % NBo;
+    const double* yr = Ys + row * NF;
+    double Zc[NC * ND];
+    sfor<NC * ND>([&](auto CD) {
+      constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+      constexpr int cx = cpx(LC, c), cy = cpy(LC, c), cz = cpz(LC, c);
+      constexpr int dx = cpx(LD, d), dy = cpy(LD, d), dz = cpz(LD, d);
+      double s = 0.0;
+      sfor<(dx + 1) * (dy + 1) * (dz + 1)>([&](auto TT) {
+        constexpr int tt = decltype(TT)::value;
+        constexpr int tx = tt / ((dy + 1) * (dz + 1)), ty = (tt / (dz + 1)) % (dy + 1), tz = tt % (dz + 1);
+        constexpr double cb = binomc(dx, tx) * binomc(dy, ty) * binomc(dz, tz);
+        constexpr int f = cindex(LC, cx + tx, cy + ty, cz + tz);
+        const double pw = cdp[0][dx - tx] * cdp[1][dy - ty] * cdp[2][dz - tz];
+        s = fma(cb * pw, yr[f], s);
+      });
+      Zc[c * ND + d] = s;
+    });
+    const double* __restrict__ DP = D + (P0 + ao) * N;
+    const double* __restrict__ DQ = D + (Q0 + bo) * N;
+    const double dpq = DP[Q0 + bo];
+    double jab = 0.0, kac[NCo], kbc[NCo], kad[NDo], kbd[NDo];
+    sfor<NCo>([&](auto I) { kac[decltype(I)::value] = kbc[decltype(I)::value] = 0.0; });
+    sfor<NDo>([&](auto I) { kad[decltype(I)::value] = kbd[decltype(I)::value] = 0.0; });
+    sfor<NCo * NDo>([&](auto OD) {
+      constexpr int co = decltype(OD)::value / NDo, dd = decltype(OD)::value % NDo;
+      double v = 0.0;
+      sfor<NC * ND>([&](auto CD) {
+        constexpr int c = decltype(CD)::value / ND, d = decltype(CD)::value % ND;
+        constexpr double cc = c2x(true, LC, co, c) * c2x(true, LD, dd, d);
+        if constexpr (cc != 0.0) v = fma(cc, Zc[c * ND + d], v);
+      });
+      const int64_t Rr = R0 + co, Ss = S0 + dd;
+      jab = fma(v, D[Rr * N + Ss], jab);
+      atomicAdd(G + Q::O1 + co * NDo + dd, v * dpq);
+      kac[co] = fma(v, DQ[Ss], kac[co]);
+      kbc[co] = fma(v, DP[Ss], kbc[co]);
+      kad[dd] = fma(v, DQ[Rr], kad[dd]);
+      kbd[dd] = fma(v, DP[Rr], kbd[dd]);
+    });
+    G[row] = jab;  // the row's lane alone writes J'_ab of its row
+    sfor<NCo>([&](auto I) {
+      constexpr int co = decltype(I)::value;
+      atomicAdd(G + Q::O2 + ao * NCo + co, kac[co]);
+      atomicAdd(G + Q::O4 + bo * NCo + co, kbc[co]);
+    });
+    sfor<NDo>([&](auto I) {
+      constexpr int dd = decltype(I)::value;
+      atomicAdd(G + Q::O3 + ao * NDo + dd, kad[dd]);
+      atomicAdd(G + Q::O5 + bo * NDo + dd, kbd[dd]);
+    });
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_epi_jk_kernel(BasisView Bv, BucketArgs A) {
+  using P = EpiCfg<LA, LB, LC, LD, true>;
+  using Q = EpiJkCfg<LA, LB, LC, LD>;
+  constexpr int NEF = P::NEF, TE = P::TE, TPB = Q::TPB;
+  const short* __restrict__ pos = ChunkCfg<LA, LB, LC, LD, 0>::pos();
+  __shared__ double sm[TPB * Q::PER];
+  __shared__ int s_key[TPB][6];   // block origins (row AO * nao + column AO)
+  __shared__ double s_w[TPB];     // degeneracy factor f of the task
+  const int team = threadIdx.x / TE;
+  const int tl = threadIdx.x % TE;
+  const int64_t t = A.t0 + (int64_t)blockIdx.x * TPB + team;
+  const bool valid = t < A.t1;
+  double* base = sm + team * Q::PER;
+  double* Ys = base + (P::XG ? 0 : NEF);
+  double* G = base + P::PER;
+  const double* X = base;
+  const int64_t N = A.nao;
+  int any = 0;
+  Task tk;
+  int64_t rs[6] = {0, 0, 0, 0, 0, 0}, cs[6] = {0, 0, 0, 0, 0, 0};
+  if (valid) {
+    tk = A.tasks[t];
+    const int64_t slot = t - A.t0;
+    any = A.flags[slot];
+    const double* src = A.scratch + slot * NEF;
+    if constexpr (P::XG) {
+      X = src;
+    } else if (any) {
+      for (int k = tl; k < NEF; k += TE) base[k] = src[pos ? pos[k] : k];
+    }
+    for (int k = tl; k < Q::MJK; k += TE) G[k] = 0.0;
+    const VShell& sa = Bv.vsh[tk.sh[0]];
+    const VShell& sb = Bv.vsh[tk.sh[1]];
+    const VShell& sc = Bv.vsh[tk.sh[2]];
+    const VShell& sd = Bv.vsh[tk.sh[3]];
+    const int64_t P0 = sa.ao, Q0 = sb.ao, R0 = sc.ao, S0 = sd.ao;
+    rs[0] = P0; cs[0] = Q0;
+    rs[1] = R0; cs[1] = S0;
+    rs[2] = P0; cs[2] = R0;
+    rs[3] = P0; cs[3] = S0;
+    rs[4] = Q0; cs[4] = R0;
+    rs[5] = Q0; cs[5] = S0;
+    if (tl == 0) {
+      for (int r = 0; r < 6; ++r) s_key[team][r] = (int)(rs[r] * N + cs[r]);
+      s_w[team] = jk_degeneracy(sa.shell, sb.shell, sc.shell, sd.shell);
+    }
+  } else if (tl == 0) {
+    for (int r = 0; r < 6; ++r) s_key[team][r] = -1 - team;  // matches no task
+    s_w[team] = 0.0;
+  }
+  __syncthreads();
+  if (valid && any) epi_bra<LA, LB, LC, LD, true>(Bv, tk, X, Ys, tl, TE, P::XG ? pos : nullptr);
+  __syncthreads();
+  if (valid && any) epi_ket_jk<LA, LB, LC, LD>(Bv, tk, Ys, G, A, tl, TE);
+  __syncthreads();
+  if (!valid) return;
+  // the first task of each run of equal block origins adds the run's sum
+  for (int m = tl; m < Q::MJK; m += TE) {
+    const int r = m < Q::O1 ? 0 : m < Q::O2 ? 1 : m < Q::O3 ? 2 : m < Q::O4 ? 3 : m < Q::O5 ? 4 : 5;
+    const int key = s_key[team][r];
+    if (team > 0 && s_key[team - 1][r] == key) continue;
+    const double jw = r < 2 ? 2.0 : 1.0;
+    double v = jw * s_w[team] * G[m];
+    for (int u = team + 1; u < TPB && s_key[u][r] == key; ++u)
+      v += jw * s_w[u] * sm[u * Q::PER + P::PER + m];
+    double* dst = r < 2 ? A.jk_j : A.jk_k;
+    if (!dst || v == 0.0) continue;
+    const int o0 = r == 0 ? 0 : r == 1 ? Q::O1 : r == 2 ? Q::O2 : r == 3 ? Q::O3 : r == 4 ? Q::O4 : Q::O5;
+    const int ncol = r == 0 ? Q::NBo : (r == 1 || r == 3 || r == 5) ? Q::NDo : Q::NCo;
+    const int i = (m - o0) / ncol, j = (m - o0) % ncol;
+    atomicAdd(dst + (rs[r] + i) * N + cs[r] + j, v);
+  }
+}
+
 // ---------------------------------------------------------------------
 // Epilogue of the i-derivative chunks (int2e_ip1, src/autocode/grad2.c:19-68):
 // per task, in LDS,
@@ -1195,6 +1483,16 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t st
   hipError_t e = hipSuccess;
   ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
+  if constexpr (DER == 0) {
+    if (A.jk_j || A.jk_k) {
+      using Q = EpiJkCfg<LA, LB, LC, LD>;
+      if (!A.sph) return hipErrorInvalidValue;
+      const dim3 egrid((unsigned)((n + Q::TPB - 1) / Q::TPB));
+      hipLaunchKernelGGL((eri_epi_jk_kernel<LA, LB, LC, LD>), egrid, dim3(Q::THREADS), 0, stream, B,
+                         A);
+      return hipGetLastError();
+    }
+  }
   if constexpr (DER == 0 || DER == 2) {
     if (A.sph) {
       using P = EpiCfg<LA, LB, LC, LD>;
@@ -1349,6 +1647,12 @@ hipError_t launch_chunk3_class(const BasisView& B, const BucketArgs& A, hipStrea
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 4>), grid, dim3(256), 0, stream, B, A); \
     else if (!A.sph)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 0, false>), grid, dim3(256), 0, stream, B, A); \
+    else if ((A.jk_j || A.jk_k) && A.fp32)                                                 \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float, 0, true, true>), grid, dim3(256), 0, \
+                         stream, B, A);                                                    \
+    else if (A.jk_j || A.jk_k)                                                             \
+      hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, double, 0, true, true>), grid, dim3(256), 0, \
+                         stream, B, A);                                                    \
     else if (A.fp32)                                                                       \
       hipLaunchKernelGGL((eri_fast_kernel<a, b, c, d, float>), grid, dim3(256), 0, stream, B, A); \
     else                                                                                   \

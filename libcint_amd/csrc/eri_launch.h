@@ -186,6 +186,8 @@ hipError_t launch_jk_grouped(const int* shls, const int64_t* off, const int* row
                              const int* cols_ptr, int ncols, const double* eri, const int* ao_loc,
                              const double* D, int nao, double* Jp, double* Kp, hipStream_t stream);
 hipError_t launch_symmetrize(const double* X, double* out, int n, hipStream_t stream);
+// x[0..n) = 0 (a kernel, so that captured graphs hold no memset node)
+hipError_t launch_zero(double* x, int64_t n, hipStream_t stream);
 
 hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, const double* eri,
                             const int* ao_loc, const double* D, int nao, double* J, double* K,

@@ -52,21 +52,13 @@ __global__ void jk_digest_kernel(const int* __restrict__ shls, const int64_t* __
 }
 
 // ---------------------------------------------------------------------
-// Grouped digest.  With the degeneracy factor f = (i==j ? 1/2 : 1)
-// (k==l ? 1/2 : 1) (ij==kl ? 1/2 : 1) of a unique quartet, the sum over its
-// 8 index maps reduces to six block updates of half matrices
-//     J'_ij += 2f (ij|kl) D_kl      J'_kl += 2f (ij|kl) D_ij
-//     K'_ik += f (ij|kl) D_jl       K'_il += f (ij|kl) D_jk
-//     K'_jk += f (ij|kl) D_il       K'_jl += f (ij|kl) D_ik
-// and J = J' + J'^T, K = K' + K'^T.  The row pass runs one workgroup per
+// Grouped digest of a stored ERI buffer: the six block updates of
+// jk_degeneracy's half matrices (device_common.h).  The row pass runs one workgroup per
 // group of quartets sharing (i, j): lane (ab, s) owns the bra element ab
 // and every S-th quartet, J'_ij stays in registers, rows i and j of K' in
 // LDS; both are flushed once per group.  The column pass does J'_kl per
 // group of quartets sharing (k, l) the same way.
 // ---------------------------------------------------------------------
-__device__ __forceinline__ double jk_degeneracy(int i, int j, int k, int l) {
-  return (i == j ? 0.5 : 1.0) * (k == l ? 0.5 : 1.0) * ((i == k && j == l) ? 0.5 : 1.0);
-}
 
 __global__ void __launch_bounds__(256) jk_rows_kernel(
     const int* __restrict__ shls, const int64_t* __restrict__ off, const int* __restrict__ gq,
@@ -191,6 +183,17 @@ __global__ void __launch_bounds__(256) jk_cols_kernel(
 }
 
 // out = X + X^T over the full matrix (each unordered pair by one thread)
+__global__ void zero_kernel(double* __restrict__ x, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = 0.0;
+}
+
+hipError_t launch_zero(double* x, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x, n);
+  return hipGetLastError();
+}
+
 __global__ void symmetrize_kernel(const double* __restrict__ X, double* __restrict__ out, int n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * n) return;

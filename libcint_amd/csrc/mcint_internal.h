@@ -138,6 +138,11 @@ struct BucketArgs {
   int nao;
   int nslot;             // gradient accumulator copies (spreads the atomics)
   int grad_stride;
+  // J/K digest (plain four-centre plans over unique quartets): instead of
+  // storing the block, add its six block updates to the half matrices
+  // J' (jk_j) and K' (jk_k), nao x nao row-major, density dm
+  double* jk_j;
+  double* jk_k;
 };
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }

@@ -43,8 +43,10 @@ class JKBuilder:
     """J and K of a basis for any symmetric density, on the device.
 
     Holds the ERI plan of the unique quartets (optionally Schwarz-screened
-    and split into an FP32 tier), the device ERI buffer and the grouped
-    digest plan.  ``run(dm)`` recomputes the ERIs (direct mode) unless
+    and split into an FP32 tier).  With fused=True (the default) ``run(dm)``
+    contracts every block with the density inside the integral kernels and
+    no ERI buffer exists; with fused=False it keeps the device ERI buffer
+    and a grouped digest plan, and ``run(dm)`` recomputes the ERIs unless
     ``recompute=False`` and the buffer is current, then digests them.
     fp32_threshold > 0 selects the mixed-precision tier: quartets whose
     Schwarz bound times dmax (a bound on |D|) is below it run their
@@ -52,7 +54,7 @@ class JKBuilder:
     """
 
     def __init__(self, basis: Basis, quartets: np.ndarray | None = None, device="cuda",
-                 fp32_threshold: float = 0.0, dmax: float = 1.0):
+                 fp32_threshold: float = 0.0, dmax: float = 1.0, fused: bool = True):
         import ctypes
         self.dev = torch.device(device)
         self.nao = nao_sph(basis)
@@ -63,20 +65,36 @@ class JKBuilder:
         if fp32_threshold > 0:
             flags = fp32_quartet_flags(self.db, q, np.array([dmax]), fp32_threshold)
         self.plan = Plan(self.db, "int2e_sph", q, fp32_flags=flags)
+        self.fused = fused
+        self.jk_handle = None
+        self.fresh = False
+        if fused:
+            self.eri = None
+            return
         self.eri = torch.empty(max(self.plan.out_size, 1), dtype=torch.float64, device=self.dev)
         h = ctypes.c_void_p()
         _check(lib().mcint_jk_plan_create(ctypes.byref(h), self.db.handle,
                                           q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                           q.shape[0], None), "mcint_jk_plan_create")
         self.jk_handle = h
-        self.fresh = False
 
     def stats(self) -> dict:
         return self.plan.stats()
 
     def compute_eri(self, stream_ptr: int = 0, graph: bool = False):
+        if self.fused:
+            raise RuntimeError("JKBuilder(fused=True) keeps no ERI buffer")
         self.plan.execute(self.eri.data_ptr(), stream_ptr=stream_ptr, graph=graph)
         self.fresh = True
+
+    def fused_jk(self, dm, with_j=True, with_k=True, stream_ptr: int = 0):
+        """J and K with the contraction inside the integral kernels."""
+        dm_t = torch.as_tensor(dm, dtype=torch.float64, device=self.dev).contiguous()
+        vj = torch.empty((self.nao, self.nao), dtype=torch.float64, device=self.dev) if with_j else None
+        vk = torch.empty((self.nao, self.nao), dtype=torch.float64, device=self.dev) if with_k else None
+        self.plan.execute_jk(dm_t.data_ptr(), self.nao, vj.data_ptr() if with_j else 0,
+                             vk.data_ptr() if with_k else 0, stream_ptr)
+        return vj, vk
 
     def digest(self, dm, with_j=True, with_k=True, stream_ptr: int = 0):
         dm_t = torch.as_tensor(dm, dtype=torch.float64, device=self.dev).contiguous()
@@ -89,6 +107,8 @@ class JKBuilder:
         return vj, vk
 
     def run(self, dm, recompute: bool = True, stream_ptr: int = 0):
+        if self.fused:
+            return self.fused_jk(dm, stream_ptr=stream_ptr)
         if recompute or not self.fresh:
             self.compute_eri(stream_ptr)
         return self.digest(dm, stream_ptr=stream_ptr)
@@ -102,17 +122,19 @@ class JKBuilder:
 
 
 def get_jk(basis: Basis, dm, nshell: int | None = None, quartets: np.ndarray | None = None,
-           device="cuda", fp32_threshold: float = 0.0, stats: dict | None = None):
+           device="cuda", fp32_threshold: float = 0.0, stats: dict | None = None,
+           fused: bool = True):
     """J and K (torch float64 on the device) for a symmetric density dm.
 
     fp32_threshold > 0 selects the mixed-precision tier: quartets whose
     Schwarz bound times max|D| is below it run their recurrences in FP32.
+    fused=False stores the ERIs and digests the buffer afterwards.
     """
     dm_np = np.asarray(dm, dtype=np.float64)
     if quartets is None and nshell is not None:
         quartets = unique_quartets(nshell)
     jb = JKBuilder(basis, quartets, device=device, fp32_threshold=fp32_threshold,
-                   dmax=float(np.abs(dm_np).max()))
+                   dmax=float(np.abs(dm_np).max()), fused=fused)
     try:
         if stats is not None:
             st = jb.stats()

@@ -67,6 +67,63 @@ def test_jk_and_energy_vs_oracle(c0_setup):
     assert abs(e - e_ref) < 1e-10 * max(1.0, abs(e_ref))
 
 
+def test_fused_jk_paths(c0_setup):
+    """J/K contracted inside the integral kernels (no ERI buffer) on every
+    kernel path: the register kernels (default), the generic kernel alone,
+    and the stored-buffer digest, each against the oracle's dense
+    contraction; J alone and K alone leave the other matrix untouched."""
+    import torch
+    import libcint_amd as mc
+    from libcint_amd import jk
+    m, eri, dm, _ = c0_setup
+    B = mc.Basis(m.atm, m.bas, m.env)
+    J = np.einsum("ijkl,kl->ij", eri, dm)
+    K = np.einsum("ijkl,jl->ik", eri, dm)
+    tol_j = 1e-11 * max(1, np.abs(J).max())
+    tol_k = 1e-11 * max(1, np.abs(K).max())
+    for label, kw, generic in (("stored", {"fused": False}, 0), ("fused", {}, 0),
+                               ("fused generic", {}, 1)):
+        _say("c0 J/K", label)
+        old = mc.tune(mc.TUNE_GENERIC_ONLY, generic)
+        try:
+            vj, vk = jk.get_jk(B, dm, **kw)
+        finally:
+            mc.tune(mc.TUNE_GENERIC_ONLY, old)
+        np.testing.assert_allclose(vj.cpu().numpy(), J, rtol=0, atol=tol_j, err_msg=label)
+        np.testing.assert_allclose(vk.cpu().numpy(), K, rtol=0, atol=tol_k, err_msg=label)
+    jb = jk.JKBuilder(B)
+    try:
+        vj, _ = jb.fused_jk(dm, with_k=False)
+        _, vk = jb.fused_jk(dm, with_j=False)
+        torch.cuda.synchronize()
+    finally:
+        jb.close()
+    np.testing.assert_allclose(vj.cpu().numpy(), J, rtol=0, atol=tol_j)
+    np.testing.assert_allclose(vk.cpu().numpy(), K, rtol=0, atol=tol_k)
+
+
+def test_fused_jk_matches_stored_on_c3():
+    """c3 (20 atoms, d shells): the fused J/K equals the stored-buffer
+    digest, in FP64 and with the FP32 tier (the same blocks, summed in
+    another order)."""
+    import libcint_amd as mc
+    from libcint_amd import jk
+    from libcint_amd.synthetic import make_molecule, random_density
+    m = make_molecule("c3")
+    B = mc.Basis(m.atm, m.bas, m.env)
+    dm = random_density(m.nao_sph(), seed=5)
+    for thr in (0.0, 1e-7):
+        _say("c3 fused vs stored J/K, fp32 threshold", thr)
+        vj_s, vk_s = jk.get_jk(B, dm, fp32_threshold=thr, fused=False)
+        vj_f, vk_f = jk.get_jk(B, dm, fp32_threshold=thr)
+        sj = max(1.0, float(vj_s.abs().max()))
+        sk = max(1.0, float(vk_s.abs().max()))
+        dj = float((vj_f - vj_s).abs().max())
+        dk = float((vk_f - vk_s).abs().max())
+        _say(f"max|dJ| {dj:.2e} max|dK| {dk:.2e}")
+        assert dj < 1e-12 * sj and dk < 1e-12 * sk
+
+
 def _small(natoms=5):
     from libcint_amd.synthetic import make_molecule, random_density, subset
     m = subset(make_molecule("c3"), natoms)
