@@ -403,11 +403,15 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * 4): mcint_plan_execute on a stream captures the plan's buckets into that
  * many concurrent graph branches (buckets balanced by estimated work, one
  * scratch region per branch); takes effect at the next capture.
+ * MCINT_TUNE_WAVE_KERNELS (0 or 1, default 1): team-of-one buckets of the
+ * chunked classes that have a wave kernel (one 64-lane wave per quartet)
+ * run it instead of their component-chunk kernels; read at launch.
  * Applies to plans created afterwards; returns the previous value, or a
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0
 #define MCINT_TUNE_GENERIC_ONLY 1
 #define MCINT_TUNE_GRAPH_BRANCHES 2
+#define MCINT_TUNE_WAVE_KERNELS 3
 int mcint_tune(int key, int value);
 /* Per-class override of MCINT_TUNE_PRIMS_PER_LANE for one integral kind
  * (intor code) and canonical class (la >= lb, lc >= ld, (la,lb) >= (lc,ld)

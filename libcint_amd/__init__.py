@@ -151,6 +151,7 @@ def lib():
 TUNE_PRIMS_PER_LANE = 0
 TUNE_GENERIC_ONLY = 1
 TUNE_GRAPH_BRANCHES = 2
+TUNE_WAVE_KERNELS = 3
 
 
 def tune(key: int, value: int) -> int:

@@ -596,6 +596,15 @@ extern "C" int mcint_tune(int key, int value) {
     g_graph_branches = value;
     return old;
   }
+  if (key == MCINT_TUNE_WAVE_KERNELS) {
+    if (value != 0 && value != 1) {
+      set_err("mcint_tune: MCINT_TUNE_WAVE_KERNELS takes 0 or 1");
+      return MCINT_ERR_ARG;
+    }
+    const int old = g_wave_kernels;
+    g_wave_kernels = value;
+    return old;
+  }
   if (key == MCINT_TUNE_GENERIC_ONLY) {
     if (value != 0 && value != 1) {
       set_err("mcint_tune: MCINT_TUNE_GENERIC_ONLY takes 0 or 1");
@@ -1038,6 +1047,7 @@ static BucketArgs bucket_args(const mcint_plan* p, const Bucket& bk, double* out
   A.grad_stride = 0;
   A.jk_j = nullptr;
   A.jk_k = nullptr;
+  A.col_major = 0;
   return A;
 }
 
@@ -2041,6 +2051,7 @@ int opt_tuple(OptDevice* d, int intor, const FINT* shls, double* blk_out) {
   A.nslot = 1;
   A.jk_j = nullptr;
   A.jk_k = nullptr;
+  A.col_major = 0;
   e = hipMemcpyAsync(c->task_d, c->task_h, sizeof(Task) * nt, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = mode == 1 ? launch_fast(B, A, c->stream)

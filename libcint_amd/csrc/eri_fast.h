@@ -13,6 +13,8 @@
 #include <type_traits>
 #include <utility>
 
+#include "gen_walk.h"
+
 namespace mcint {
 
 namespace ct {
@@ -1034,7 +1036,9 @@ __global__ void __launch_bounds__(256) eri_epi_kernel(BasisView Bv, BucketArgs A
     if constexpr (P::XG) {
       X = src;
     } else if (any) {
-      for (int k = tl; k < NEF; k += TE) sm[team * P::PER + k] = src[pos ? pos[k] : k];
+      for (int k = tl; k < NEF; k += TE)
+        sm[team * P::PER + k] =
+            src[A.col_major ? (k % C::NF) * C::NE + k / C::NF : (pos ? pos[k] : k)];
     }
   }
   __syncthreads();
@@ -1169,7 +1173,10 @@ __global__ void __launch_bounds__(256) eri_epi_jk_kernel(BasisView Bv, BucketArg
     if constexpr (P::XG) {
       X = src;
     } else if (any) {
-      for (int k = tl; k < NEF; k += TE) base[k] = src[pos ? pos[k] : k];
+      for (int k = tl; k < NEF; k += TE) {
+        using C = FastClass<LA, LB, LC, LD>;
+        base[k] = src[A.col_major ? (k % C::NF) * C::NE + k / C::NF : (pos ? pos[k] : k)];
+      }
     }
     for (int k = tl; k < Q::MJK; k += TE) G[k] = 0.0;
     const VShell& sa = Bv.vsh[tk.sh[0]];
@@ -1459,6 +1466,242 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
   }
 }
 
+// ---------------------------------------------------------------------
+// Wave kernel: one 64-lane wave per quartet for chunked classes whose
+// quartets carry few primitive quartets (the team-of-one buckets).  The
+// wave screens 64 candidate primitive quartets at once (one per lane, in
+// libcint's order), compacts the survivors and, a batch of PB at a time,
+// evaluates all their roots (one (quartet, root) per lane), then all their
+// 2D recurrence tables (one (quartet, direction, root) per lane) into LDS,
+// so that the memory latency of pair data and root tables is paid once per
+// batch; lane l owns the rows e = l and l + 64 of [e0|f0] with all NF
+// columns in registers, and per root reads the three recurrence rows
+// V_d(e_d, 0..F) once for its NF products.
+// Against the chunk kernels: no recurrence recomputed per chunk, one launch
+// instead of one per chunk, a few dozen registers instead of the whole
+// register file, so that many waves hide the recurrence's latency.  It
+// writes the block column-major (coalesced) and the chunked epilogue, told
+// so by BucketArgs::col_major, follows.
+// ---------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+struct WaveCfg {
+  using C = FastClass<LA, LB, LC, LD>;
+  static constexpr int ROWS = (C::NE + 63) / 64;
+  static constexpr int VS = (C::E + 1) * (C::F + 1);
+  static constexpr int PV = 3 * C::NR * VS;  // recurrence tables of one primitive quartet
+  // primitive quartets whose tables share the LDS at once (20 KB of tables:
+  // two waves per SIMD)
+  static constexpr int PB = 2560 / PV < 1 ? 1 : (2560 / PV > 64 ? 64 : 2560 / PV);
+  // rows that fill most of the wave, columns enough to outweigh the walk,
+  // accumulators that leave room for two waves per SIMD
+  static constexpr bool OK = C::NE >= 40 && C::NF >= 16 && ROWS * C::NF <= 96 &&
+                             !EpiCfg<LA, LB, LC, LD, true>::XG;
+};
+
+// Per primitive quartet of a wave-kernel batch: the data of its roots and
+// recurrence (plain and long-range operators).
+struct WavePrim {
+  double aij, akl, a0, a1, x, fac, theta;
+  double pq[3], pa[3], qc[3];
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A) {
+  using C = FastClass<LA, LB, LC, LD>;
+  using W = WaveCfg<LA, LB, LC, LD>;
+  constexpr int NE = C::NE, NF = C::NF, E = C::E, F = C::F, NR = C::NR;
+  constexpr int ROWS = W::ROWS, VS = W::VS, PV = W::PV, PB = W::PB;
+  constexpr int NEF = NE * NF;
+  __shared__ double s_V[PB * PV];
+  __shared__ double s_u[PB * NR], s_w[PB * NR];
+  __shared__ WavePrim s_p[PB];
+  const int64_t t = A.t0 + blockIdx.x;
+  if (t >= A.t1) return;  // the whole workgroup
+  const int lane = threadIdx.x;
+  const Task tk = A.tasks[t];
+  const VShell& sa = Bv.vsh[tk.sh[0]];
+  const VShell& sb = Bv.vsh[tk.sh[1]];
+  const VShell& sc = Bv.vsh[tk.sh[2]];
+  const VShell& sd = Bv.vsh[tk.sh[3]];
+  // this lane's rows: Cartesian exponents in the row list (shells LA..E)
+  int rx[ROWS], ry[ROWS], rz[ROWS];
+  bool has[ROWS];
+  sfor<ROWS>([&](auto S) {
+    constexpr int sl = decltype(S)::value;
+    const int e = lane + 64 * sl;
+    has[sl] = e < NE;
+    const int ee = has[sl] ? e : 0;
+    rx[sl] = cpx(LA, ee);
+    ry[sl] = cpy(LA, ee);
+    rz[sl] = cpz(LA, ee);
+  });
+  double acc[ROWS][NF];
+  sfor<ROWS>([&](auto S) {
+    sfor<NF>([&](auto J) { acc[decltype(S)::value][decltype(J)::value] = 0.0; });
+  });
+
+  // libcint's screening, as in gen_walk for plain four-centre integrals
+  // (src/cint2e.c:57-269 with the optimizer's cutoffs)
+  const int nij = sa.nprim * sb.nprim, nkl = sc.nprim * sd.nprim;
+  const int64_t pab = (int64_t)tk.sh[0] * Bv.nv + tk.sh[1];
+  const int64_t pcd = (int64_t)tk.sh[2] * Bv.nv + tk.sh[3];
+  const int64_t oij = Bv.pair_off[pab], okl = Bv.pair_off[pcd];
+  const PairPrim* __restrict__ pij = Bv.pp + oij;
+  const PairPrim* __restrict__ pkl = Bv.pp + okl;
+  const int* __restrict__ pos_ij = Bv.pp_pos + oij;
+  const int* __restrict__ pos_kl = Bv.pp_pos + okl;
+  const double dij = (LA + LB) * Bv.pair_ld[pab], dkl = (LC + LD) * Bv.pair_ld[pcd];
+  const double Ecut = Bv.expcutoff_4c;
+  const double omega = Bv.omega;
+  constexpr double cfac = 34.986836655249725 *
+                          (LA == 0 ? 0.282094791773878143 : LA == 1 ? 0.488602511902919921 : 1.0) *
+                          (LB == 0 ? 0.282094791773878143 : LB == 1 ? 0.488602511902919921 : 1.0) *
+                          (LC == 0 ? 0.282094791773878143 : LC == 1 ? 0.488602511902919921 : 1.0) *
+                          (LD == 0 ? 0.282094791773878143 : LD == 1 ? 0.488602511902919921 : 1.0);
+  // pair emptiness (src/cint2e.c:96-109)
+  int live_ij = 0, live_kl = 0;
+  for (int p = lane; p < nij; p += 64) live_ij |= (pij[p].cce0 - dij) < Ecut;
+  for (int p = lane; p < nkl; p += 64) live_kl |= (pkl[p].cce0 - dkl) < Ecut;
+  const bool live = __ballot(live_ij) != 0 && __ballot(live_kl) != 0;
+  int any = 0;
+  if (live) {
+    for (int base = 0; base < nij * nkl; base += 64) {
+      // one candidate per lane, in libcint's order (ip fastest)
+      const int p = base + lane;
+      bool pass = false;
+      WavePrim wp;
+      if (p < nij * nkl) {
+        const PairPrim P = pij[pos_ij[p % nij]];
+        const PairPrim Q = pkl[pos_kl[p / nij]];
+        const double cce_kl = Q.cce0 - dkl, cce_ij = P.cce0 - dij;
+        if (!(cce_kl > Ecut) && !(cce_ij > Ecut - cce_kl)) {
+          any = 1;
+          pass = cce_ij < Ecut && cce_kl < Ecut;  // else the stored exp factor is 0
+        }
+        if (pass) {
+          wp.aij = P.aij;
+          wp.akl = Q.aij;
+          wp.a1 = P.aij * Q.aij;
+          wp.a0 = wp.a1 * rcp_pos(P.aij + Q.aij);
+          wp.pq[0] = P.px - Q.px;
+          wp.pq[1] = P.py - Q.py;
+          wp.pq[2] = P.pz - Q.pz;
+          wp.pa[0] = P.px - sa.r[0];
+          wp.pa[1] = P.py - sa.r[1];
+          wp.pa[2] = P.pz - sa.r[2];
+          wp.qc[0] = Q.px - sc.r[0];
+          wp.qc[1] = Q.py - sc.r[1];
+          wp.qc[2] = Q.pz - sc.r[2];
+          const double rr = wp.pq[0] * wp.pq[0] + wp.pq[1] * wp.pq[1] + wp.pq[2] * wp.pq[2];
+          wp.fac = sqrt_pos(wp.a0 * rcp_pos(wp.a1 * wp.a1 * wp.a1)) * cfac * P.kab * Q.kab;
+          wp.x = wp.a0 * rr;
+          wp.theta = 1.0;
+          if (omega > 0) {
+            wp.theta = omega * omega * rcp_pos(omega * omega + wp.a0);
+            wp.x *= wp.theta;
+            wp.fac *= sqrt_pos(wp.theta);
+          }
+        }
+      }
+      const unsigned long long mask = __ballot(pass);
+      const int nsurv = __popcll(mask);
+      const int slot = __popcll(mask & ((1ull << lane) - 1));
+      for (int b0 = 0; b0 < nsurv; b0 += PB) {
+        const int np = nsurv - b0 < PB ? nsurv - b0 : PB;
+        if (pass && slot >= b0 && slot < b0 + PB) s_p[slot - b0] = wp;
+        __syncthreads();
+        // roots and weights of every (primitive quartet, root)
+        for (int it = lane; it < np * NR; it += 64) {
+          const int q = it / NR, r = it - q * NR;
+          double u, w;
+          rys_root_one(NR, r, s_p[q].x, u, w);
+          if (omega > 0) {
+            const double ut = u * s_p[q].theta;
+            u = ut * rcp_pos(u + 1.0 - ut);
+          }
+          s_u[it] = u;
+          s_w[it] = w * s_p[q].fac;
+        }
+        __syncthreads();
+        // 2D recurrence of every (primitive quartet, direction, root)
+        for (int it = lane; it < np * 3 * NR; it += 64) {
+          const int q = it / (3 * NR), dr = it - q * 3 * NR, d = dr / NR, r = dr - d * NR;
+          const WavePrim& P = s_p[q];
+          const double u2 = P.a0 * s_u[q * NR + r];
+          const double tmp4 = 0.5 * rcp_pos(u2 * (P.aij + P.akl) + P.a1);
+          const double tmp5 = u2 * tmp4;
+          const double b00 = tmp5, b10 = tmp5 + tmp4 * P.akl, b01 = tmp5 + tmp4 * P.aij;
+          const double c00 = P.pa[d] - 2.0 * tmp5 * P.akl * P.pq[d];
+          const double c0p = P.qc[d] + 2.0 * tmp5 * P.aij * P.pq[d];
+          double* V = s_V + q * PV + dr * VS;
+          V[0] = d == 2 ? s_w[q * NR + r] : 1.0;
+          sfor<E>([&](auto EE) {
+            constexpr int e = decltype(EE)::value;
+            double v = c00 * V[e * (F + 1)];
+            if constexpr (e > 0) v += e * b10 * V[(e - 1) * (F + 1)];
+            V[(e + 1) * (F + 1)] = v;
+          });
+          sfor<F>([&](auto FF) {
+            constexpr int f = decltype(FF)::value;
+            sfor<E + 1>([&](auto EE) {
+              constexpr int e = decltype(EE)::value;
+              double v = c0p * V[e * (F + 1) + f];
+              if constexpr (f > 0) v += f * b01 * V[e * (F + 1) + f - 1];
+              if constexpr (e > 0) v += e * b00 * V[(e - 1) * (F + 1) + f];
+              V[e * (F + 1) + f + 1] = v;
+            });
+          });
+        }
+        __syncthreads();
+        // [e0|f0] products of this lane's rows
+        for (int q = 0; q < np; ++q) {
+          for (int r = 0; r < NR; ++r) {
+            const double* Vx = s_V + q * PV + (0 * NR + r) * VS;
+            const double* Vy = s_V + q * PV + (1 * NR + r) * VS;
+            const double* Vz = s_V + q * PV + (2 * NR + r) * VS;
+            sfor<ROWS>([&](auto S) {
+              constexpr int sl = decltype(S)::value;
+              if (has[sl]) {
+                double vx[F + 1], vy[F + 1], vz[F + 1];
+                sfor<F + 1>([&](auto K) {
+                  constexpr int k = decltype(K)::value;
+                  vx[k] = Vx[rx[sl] * (F + 1) + k];
+                  vy[k] = Vy[ry[sl] * (F + 1) + k];
+                  vz[k] = Vz[rz[sl] * (F + 1) + k];
+                });
+                sfor<NF>([&](auto J) {
+                  constexpr int j = decltype(J)::value;
+                  constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
+                  acc[sl][j] = fma(vx[fx] * vy[fy], vz[fz], acc[sl][j]);
+                });
+              }
+            });
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  any = __ballot(any) != 0 && live;
+  // column-major ([f][e]): for each column the lanes' rows are contiguous
+  const int64_t slot = t - A.t0;
+  double* __restrict__ dst = A.scratch + slot * NEF;
+  sfor<ROWS>([&](auto S) {
+    constexpr int sl = decltype(S)::value;
+    if (has[sl]) {
+      const int e = lane + 64 * sl;
+      sfor<NF>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        dst[j * NE + e] = acc[sl][j];
+      });
+    }
+  });
+  if (lane == 0) {
+    A.flags[slot] = any;
+    if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
+  }
+}
+
 template <int LA, int LB, int LC, int LD, int CH, int DER>
 hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t stream, dim3 grid) {
   constexpr int TH = ChunkCfg<LA, LB, LC, LD, DER>::THREADS;
@@ -1475,13 +1718,25 @@ hipError_t launch_one_chunk(const BasisView& B, const BucketArgs& A, hipStream_t
 }
 
 template <int LA, int LB, int LC, int LD, int DER, int... CHs>
-hipError_t launch_chunks(const BasisView& B, const BucketArgs& A, hipStream_t stream,
+hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t stream,
                          std::integer_sequence<int, CHs...>) {
-  const int64_t n = A.t1 - A.t0;
-  const int64_t per = ChunkCfg<LA, LB, LC, LD, DER>::THREADS / A.team;
+  const int64_t n = A0.t1 - A0.t0;
+  const int64_t per = ChunkCfg<LA, LB, LC, LD, DER>::THREADS / A0.team;
   const dim3 grid((unsigned)((n + per - 1) / per));
   hipError_t e = hipSuccess;
-  ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
+  BucketArgs A = A0;
+  if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK) {
+    // team-of-one buckets (few primitive quartets per quartet), plain
+    // operator and long range, FP64
+    if (A.team == 1 && B.omega >= 0 && !A.fp32 && g_wave_kernels) {
+      A.col_major = 1;
+      hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)n), dim3(64), 0, stream,
+                         B, A);
+      e = hipGetLastError();
+    }
+  }
+  if (!A.col_major)
+    ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
   if constexpr (DER == 0) {
     if (A.jk_j || A.jk_k) {

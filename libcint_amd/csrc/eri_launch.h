@@ -198,6 +198,10 @@ hipError_t launch_jk_digest(const int* shls, const int64_t* off, int64_t nq, con
 // an LDS epilogue (scratch of chunked_block_doubles() per task), 0 = none.
 // Cartesian output: every kind except short range.
 int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld, double omega);
+// 1: team-of-one buckets of the chunked classes with a wave kernel run it
+// instead of their chunk kernels (mcint_tune MCINT_TUNE_WAVE_KERNELS;
+// read at launch, so graphs captured earlier keep their kernels)
+extern int g_wave_kernels;
 int chunked_block_doubles(int kind, int la, int lb, int lc, int ld);
 int fast_team_lanes_max();
 hipError_t launch_fast(const BasisView& B, const BucketArgs& A, hipStream_t stream);

@@ -131,6 +131,8 @@ struct BucketArgs {
   int sph;
   int team;              // lanes per task (fast kernels)
   int fp32;              // recurrences and contraction in FP32 (fast kernels)
+  int col_major;         // chunked classes: per-task blocks stored [f][e] (wave
+                         // kernel) instead of in the chunk order
   // gradient digest (i-derivative plans): instead of storing the block,
   // contract it with the density and add to grad[atom of shell i][x]
   const double* dm;      // nao x nao density, row-major (nullable: store mode)

@@ -108,6 +108,54 @@ def test_stratified_c1_int2e_sph(mc, orc):
     _check_stratified(mc, orc, m, "int2e_sph", q, seed=1)
 
 
+def test_wave_kernels_match_chunk_kernels_c1(mc):
+    """The team-of-one buckets of c1's chunked classes run on the wave
+    kernel (one wave per quartet); up to 20,000 quartets of each such bucket
+    give the same blocks with the component-chunk kernels (the other path of
+    the same buckets), to the parity bar of the oracle comparisons
+    (1e-12 + 1e-10 |x|; the two paths sum primitives in different orders),
+    plain and long-range operator."""
+    import torch
+    from libcint_amd.synthetic import make_molecule, screened_quartets
+    m = make_molecule("c1")
+    B = mc.Basis(m.atm, m.bas, m.env)
+    db = mc.DeviceBasis(B)
+    q, _ = screened_quartets(db.schwarz(m.nbas), m.nbas, 1e-12)
+    plan = mc.Plan(db, "int2e_sph", q)
+    rng = np.random.default_rng(5)
+    picks = []
+    for i, b in enumerate(plan.buckets()):
+        if b["team"] == 1 and b["path"] == "chunked":
+            ids = plan.bucket_tasks(i, 0, int(b["tasks"])).astype(np.int64)
+            picks.append(np.sort(rng.choice(ids, size=min(20000, ids.size), replace=False)))
+    plan.close()
+    db.close()
+    sub = q[np.unique(np.concatenate(picks))]
+    for omega in (0.0, 0.4):
+        env = m.env.copy()
+        env[8] = omega
+        dbo = mc.DeviceBasis(mc.Basis(m.atm, m.bas, env))
+        pl = mc.Plan(dbo, "int2e_sph", sub)
+        outs = []
+        for wave in (1, 0):
+            old = mc.tune(mc.TUNE_WAVE_KERNELS, wave)
+            try:
+                o = torch.full((pl.out_size,), float("nan"), dtype=torch.float64, device="cuda")
+                pl.execute(o.data_ptr(), graph=False)
+                torch.cuda.synchronize()
+                outs.append(o.cpu().numpy())
+            finally:
+                mc.tune(mc.TUNE_WAVE_KERNELS, old)
+        pl.close()
+        dbo.close()
+        _say("wave vs chunk kernels, omega", omega, ":", len(sub), "quartets,", outs[0].size, "ERIs")
+        assert np.isfinite(outs[0]).all() and np.isfinite(outs[1]).all()
+        err = np.abs(outs[0] - outs[1])
+        _say("max |difference|", float(err.max()), "max relative",
+             float((err / np.maximum(np.abs(outs[1]), 1e-300)).max()))
+        assert (err <= 1e-12 + 1e-10 * np.abs(outs[1])).all(), float(err.max())
+
+
 def test_stratified_c3_ip1_and_cart(mc, orc):
     from libcint_amd.jk import gradient_quartets
     from libcint_amd.synthetic import make_molecule, unique_quartets
