@@ -420,6 +420,9 @@ def main():
     ap.add_argument("--only-buckets", default="",
                     help="run only the buckets of these classes, e.g. '3221:1,3321:1' "
                          "(la lb lc ld : team), --steps times each, then exit (for counter runs)")
+    ap.add_argument("--wave-kernels", type=int, default=1, choices=(0, 1, 2),
+                    help="team-of-one chunked buckets on the wave kernel: 0 never, 1 the "
+                         "classes measured faster (default), 2 every class that has one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -445,6 +448,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
         return
+    mc.tune(mc.TUNE_WAVE_KERNELS, args.wave_kernels)
     m = make_molecule(args.config)
     nshell = m.nbas if m.aux_start < 0 else m.aux_start
     B = mc.Basis(m.atm, m.bas, m.env)

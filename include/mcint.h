@@ -403,9 +403,10 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * 4): mcint_plan_execute on a stream captures the plan's buckets into that
  * many concurrent graph branches (buckets balanced by estimated work, one
  * scratch region per branch); takes effect at the next capture.
- * MCINT_TUNE_WAVE_KERNELS (0 or 1, default 1): team-of-one buckets of the
- * chunked classes that have a wave kernel (one 64-lane wave per quartet)
- * run it instead of their component-chunk kernels; read at launch.
+ * MCINT_TUNE_WAVE_KERNELS (default 1): team-of-one buckets of the chunked
+ * classes run the wave kernel (one 64-lane wave per quartet) instead of
+ * their component-chunk kernels: 0 never, 1 for the classes measured
+ * faster, 2 for every class that has one; read at launch.
  * Applies to plans created afterwards; returns the previous value, or a
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0

@@ -597,8 +597,8 @@ extern "C" int mcint_tune(int key, int value) {
     return old;
   }
   if (key == MCINT_TUNE_WAVE_KERNELS) {
-    if (value != 0 && value != 1) {
-      set_err("mcint_tune: MCINT_TUNE_WAVE_KERNELS takes 0 or 1");
+    if (value < 0 || value > 2) {
+      set_err("mcint_tune: MCINT_TUNE_WAVE_KERNELS takes 0, 1 or 2");
       return MCINT_ERR_ARG;
     }
     const int old = g_wave_kernels;

@@ -1492,10 +1492,14 @@ struct WaveCfg {
   // primitive quartets whose tables share the LDS at once (20 KB of tables:
   // two waves per SIMD)
   static constexpr int PB = 2560 / PV < 1 ? 1 : (2560 / PV > 64 ? 64 : 2560 / PV);
-  // rows that fill most of the wave, columns enough to outweigh the walk,
-  // accumulators that leave room for two waves per SIMD
-  static constexpr bool OK = C::NE >= 40 && C::NF >= 16 && ROWS * C::NF <= 96 &&
+  // compiled: accumulators that leave room for two waves per SIMD
+  static constexpr bool OK = C::NE >= 16 && ROWS * C::NF <= 96 &&
                              !EpiCfg<LA, LB, LC, LD, true>::XG;
+  // used by default: blocks of 480+ components, measured faster than the
+  // chunk kernels on every such c1 bucket ((dd|dd) 1.6 -> 1.0 ms, (fd|fd)
+  // 3.5 -> 1.8 ms, (fd|dp) 9.1 -> 6.7 ms); below, the walk outweighs the
+  // products and the chunk kernels win ((fp|ps) 2.9 -> 11.8 ms)
+  static constexpr bool DEFAULT = OK && C::NE * C::NF >= 480;
 };
 
 // Per primitive quartet of a wave-kernel batch: the data of its roots and
@@ -1728,7 +1732,8 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
   if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK) {
     // team-of-one buckets (few primitive quartets per quartet), plain
     // operator and long range, FP64
-    if (A.team == 1 && B.omega >= 0 && !A.fp32 && g_wave_kernels) {
+    if (A.team == 1 && B.omega >= 0 && !A.fp32 &&
+        (g_wave_kernels == 2 || (g_wave_kernels == 1 && WaveCfg<LA, LB, LC, LD>::DEFAULT))) {
       A.col_major = 1;
       hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)n), dim3(64), 0, stream,
                          B, A);
