@@ -420,6 +420,15 @@ int mcint_tune(int key, int value);
  * restores the global value.  Written by the autotuner
  * (libcint_amd/autotune.py).  Returns the previous override. */
 int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int prims_per_lane);
+/* Per-class kernel choice for the team-of-one buckets of a plain int2e
+ * class (canonical order): mode 2 the wave kernel (one 64-lane wave per
+ * quartet; only where mcint_wave_kernel_exists), 1 the component-chunk
+ * kernels, 0 the built-in rule (wave kernel for blocks of 480+ [e0|f0]
+ * components).  Applies while MCINT_TUNE_WAVE_KERNELS is 1; read at launch.
+ * Returns the previous mode or a negative status.  (No libcint
+ * counterpart: the autotuner's second dimension.) */
+int mcint_tune_class_wave(int la, int lb, int lc, int ld, int mode);
+int mcint_wave_kernel_exists(int la, int lb, int lc, int ld);
 
 const char *mcint_last_error(void);
 

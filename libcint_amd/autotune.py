@@ -1,4 +1,5 @@
-"""Per-class autotuner for the register kernels' team size.
+"""Per-class autotuner: team size of the register kernels, and the kernel
+of the team-of-one buckets.
 
 A team of lanes shares one contracted quartet; its size follows from the
 number of primitive quartets each lane should handle (``mcint_tune`` /
@@ -10,7 +11,15 @@ it is measured: for each candidate the plan is rebuilt and every bucket is
 timed alone with HIP events, the times are summed per class, and each class
 keeps its fastest candidate.
 
-    python -m libcint_amd.autotune out.json      # c1 / c3 / c4 workloads
+Second dimension (plain int2e): a chunked class's team-of-one bucket runs
+either its component-chunk kernels or the wave kernel (one wave per
+quartet, ``mcint_tune_class_wave``); both are timed per bucket and the class
+keeps the faster one (``"wave"`` in the table: 2 wave kernel, 1 chunk
+kernels).
+
+    python -m libcint_amd.autotune out.json           # c1 / c3 / c4 workloads
+    python -m libcint_amd.autotune --wave-only out.json   # kernel choice only,
+                                                      # on top of the committed table
 
 The committed table (``libcint_amd/tuning/mi355x.json``) was produced this
 way on an MI355X and is applied when the library is loaded
@@ -25,7 +34,8 @@ import sys
 
 import numpy as np
 
-from . import INTOR_CODES, TUNE_PRIMS_PER_LANE, DeviceBasis, Plan, tune, tune_class
+from . import (INTOR_CODES, TUNE_PRIMS_PER_LANE, TUNE_WAVE_KERNELS, DeviceBasis, Plan, tune,
+               tune_class, tune_class_wave, wave_kernel_exists)
 
 CANDIDATES = (8, 16, 32, 64, 128)
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "mi355x.json")
@@ -84,6 +94,44 @@ def autotune(dbasis: DeviceBasis, name: str, quartets, candidates=CANDIDATES, re
     return table
 
 
+def autotune_wave(dbasis: DeviceBasis, quartets, reps: int = 2, margin: float = 0.03,
+                  log=print) -> dict:
+    """Kernel of every chunked int2e_sph class's team-of-one bucket: the
+    wave kernel where it is at least `margin` faster than the chunk kernels,
+    else the chunk kernels ({"la,lb,lc,ld": 2 or 1})."""
+    import torch
+    plan = Plan(dbasis, "int2e_sph", quartets)
+    out = torch.empty(max(plan.out_size, 1), dtype=torch.float64, device="cuda")
+    one = [(i, b) for i, b in enumerate(plan.buckets())
+           if b["team"] == 1 and b["path"] == "chunked" and wave_kernel_exists(b["cls"])]
+    times = {}
+    for mode in (1, 2):
+        for _, b in one:
+            tune_class_wave(b["cls"], mode)
+        plan.execute(out.data_ptr(), graph=False)
+        torch.cuda.synchronize()
+        for i, b in one:
+            best = float("inf")
+            for _ in range(reps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                plan.execute_bucket(i, out.data_ptr())
+                e1.record()
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1) * 1e-3)
+            times.setdefault(tuple(b["cls"]), {})[mode] = best
+    plan.close()
+    table = {}
+    for cls, by in sorted(times.items()):
+        choice = 2 if by[2] < (1.0 - margin) * by[1] else 1
+        tune_class_wave(cls, 0)
+        table[",".join(str(c) for c in cls)] = choice
+        log(f"[autotune] wave {cls}: chunk kernels {by[1] * 1e3:.3f} ms, wave kernel "
+            f"{by[2] * 1e3:.3f} ms -> {'wave' if choice == 2 else 'chunk'}")
+    return table
+
+
 def reset(name: str) -> None:
     """Drop every per-class override of one integral kind."""
     for cls in np.ndindex(7, 7, 7, 7):
@@ -91,8 +139,15 @@ def reset(name: str) -> None:
 
 
 def apply(tables: dict) -> None:
-    """Install {intor name: {"la,lb,lc,ld": prims per lane}} overrides."""
+    """Install {intor name: {"la,lb,lc,ld": prims per lane}} overrides and,
+    under "wave", {"la,lb,lc,ld": 2 wave kernel | 1 chunk kernels}."""
     for name, table in tables.items():
+        if name == "wave":
+            for cls, mode in table.items():
+                c = [int(v) for v in cls.split(",")]
+                if int(mode) != 2 or wave_kernel_exists(c):
+                    tune_class_wave(c, int(mode))
+            continue
         if name not in INTOR_CODES:
             continue
         for cls, ppl in table.items():
@@ -134,11 +189,21 @@ def _workloads():
 
 
 def main(argv):
-    out_path = argv[1] if len(argv) > 1 else DEFAULT_TABLE
-    tables = {}
+    wave_only = "--wave-only" in argv
+    args = [a for a in argv[1:] if a != "--wave-only"]
+    out_path = args[0] if args else DEFAULT_TABLE
+    tables = load() if wave_only else {}
+    tune(TUNE_WAVE_KERNELS, 1)
     for name, db, q in _workloads():
-        print(f"[autotune] {name}: {q.shape[0]} quartets", flush=True)
-        tables[name] = autotune(db, name, q, log=lambda s: print(s, flush=True))
+        if name == "int2e_sph":
+            if not wave_only:
+                print(f"[autotune] {name}: {q.shape[0]} quartets", flush=True)
+                tables[name] = autotune(db, name, q, log=lambda s: print(s, flush=True))
+            apply({name: tables.get(name, {})})
+            tables["wave"] = autotune_wave(db, q, log=lambda s: print(s, flush=True))
+        elif not wave_only:
+            print(f"[autotune] {name}: {q.shape[0]} quartets", flush=True)
+            tables[name] = autotune(db, name, q, log=lambda s: print(s, flush=True))
         db.close()
     with open(out_path, "w") as fh:
         json.dump({"device": "MI355X (gfx950)", "candidates": list(CANDIDATES),

@@ -490,12 +490,7 @@ int g_graph_branches = 4;
 constexpr int MAX_GRAPH_BRANCHES = 8;
 // per-class overrides (mcint_tune_class), 0 = the global value; indexed by
 // integral kind and canonical class
-constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
 int16_t g_class_ppl[NKIND][NCLS] = {};
-
-inline int class_index(int la, int lb, int lc, int ld) {
-  return ((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld;
-}
 
 int pick_team(int64_t nprim_quartets, int ppl) {
   int t = 1;
@@ -575,6 +570,28 @@ extern "C" int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int p
   const int old = v;
   v = (int16_t)prims_per_lane;
   return old;
+}
+
+extern "C" int mcint_tune_class_wave(int la, int lb, int lc, int ld, int mode) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > LMAX || lb > LMAX || lc > LMAX || ld > LMAX ||
+      mode < 0 || mode > 2) {
+    set_err("mcint_tune_class_wave: bad argument");
+    return MCINT_ERR_ARG;
+  }
+  if (mode == 2 && !wave_kernel_exists(la, lb, lc, ld)) {
+    set_err("mcint_tune_class_wave: the class has no wave kernel");
+    return MCINT_ERR_UNSUPPORTED;
+  }
+  int8_t& v = g_class_wave[class_index(la, lb, lc, ld)];
+  const int old = v;
+  v = (int8_t)mode;
+  return old;
+}
+
+extern "C" int mcint_wave_kernel_exists(int la, int lb, int lc, int ld) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > LMAX || lb > LMAX || lc > LMAX || ld > LMAX)
+    return 0;
+  return wave_kernel_exists(la, lb, lc, ld);
 }
 
 extern "C" int mcint_tune(int key, int value) {

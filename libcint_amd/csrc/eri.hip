@@ -807,5 +807,13 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
 
 namespace mcint {
 int g_wave_kernels = 1;
+int8_t g_class_wave[NCLS] = {};
+#define MC_WAVE_HAS(a, b, c, d) \
+  if (la == a && lb == b && lc == c && ld == d) return WaveCfg<a, b, c, d>::OK ? 1 : 0;
+int wave_kernel_exists(int la, int lb, int lc, int ld) {
+  MCINT_CHUNK_CLASSES(MC_WAVE_HAS)
+  return 0;
+}
+#undef MC_WAVE_HAS
 MCINT_FAST_DISPATCH
 }  // namespace mcint

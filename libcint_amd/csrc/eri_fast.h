@@ -1732,8 +1732,10 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
   if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK) {
     // team-of-one buckets (few primitive quartets per quartet), plain
     // operator and long range, FP64
+    const int cw = g_class_wave[class_index(LA, LB, LC, LD)];
     if (A.team == 1 && B.omega >= 0 && !A.fp32 &&
-        (g_wave_kernels == 2 || (g_wave_kernels == 1 && WaveCfg<LA, LB, LC, LD>::DEFAULT))) {
+        (g_wave_kernels == 2 ||
+         (g_wave_kernels == 1 && (cw ? cw == 2 : WaveCfg<LA, LB, LC, LD>::DEFAULT)))) {
       A.col_major = 1;
       hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)n), dim3(64), 0, stream,
                          B, A);

@@ -73,3 +73,20 @@ def test_python_package_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(libcint_amd, "_lib", None)
     with pytest.raises(libcint_amd.McintError):
         libcint_amd.lib()
+
+
+def test_wave_kernel_choice_is_validated_on_the_host(libpath):
+    """mcint_wave_kernel_exists names the classes with a wave kernel, and
+    mcint_tune_class_wave refuses the wave kernel for a class without one
+    (host-side calls, no GPU)."""
+    import libcint_amd as mc
+    mc.lib()
+    assert mc.wave_kernel_exists((3, 2, 2, 1))       # (fd|dp): 46 rows x 16 columns
+    assert not mc.wave_kernel_exists((1, 0, 0, 0))   # single-chunk class
+    assert not mc.wave_kernel_exists((3, 3, 3, 3))   # 2 x 74 accumulators per lane: no
+    with pytest.raises(mc.McintError):
+        mc.tune_class_wave((1, 0, 0, 0), 2)
+    old = mc.tune_class_wave((3, 2, 2, 1), 2)
+    assert mc.tune_class_wave((3, 2, 2, 1), old) == 2
+    with pytest.raises(mc.McintError):
+        mc.tune_class_wave((3, 2, 2, 1), 3)
