@@ -1467,16 +1467,18 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
 }
 
 // ---------------------------------------------------------------------
-// Wave kernel: one 64-lane wave per quartet for chunked classes whose
-// quartets carry few primitive quartets (the team-of-one buckets).  The
-// wave screens 64 candidate primitive quartets at once (one per lane, in
-// libcint's order), compacts the survivors and, a batch of PB at a time,
-// evaluates all their roots (one (quartet, root) per lane), then all their
-// 2D recurrence tables (one (quartet, direction, root) per lane) into LDS,
-// so that the memory latency of pair data and root tables is paid once per
-// batch; lane l owns the rows e = l and l + 64 of [e0|f0] with all NF
+// Wave kernel: LPT = 16, 32 or 64 lanes per quartet (a wave holds 64/LPT
+// quartets) for chunked classes whose quartets carry few primitive
+// quartets (the team-of-one buckets).  A quartet's lanes screen LPT
+// candidate primitive quartets at once (one per lane, in libcint's order),
+// compact the survivors and, a batch of PB at a time, evaluate all their
+// roots (one (primitive quartet, root) per lane), then all their 2D
+// recurrence tables (one (primitive quartet, direction, root) per lane) into
+// LDS, so that the memory latency of pair data and root tables is paid once
+// per batch; lane l owns the rows e = l (and l + LPT) of [e0|f0] with all NF
 // columns in registers, and per root reads the three recurrence rows
-// V_d(e_d, 0..F) once for its NF products.
+// V_d(e_d, 0..F) once for its NF products.  Loop bounds are maxima over the
+// wave's quartets, so every barrier is reached by the whole wave.
 // Against the chunk kernels: no recurrence recomputed per chunk, one launch
 // instead of one per chunk, a few dozen registers instead of the whole
 // register file, so that many waves hide the recurrence's latency.  It
@@ -1486,19 +1488,23 @@ __global__ void __launch_bounds__(256) eri_ip1_epi_kernel(BasisView Bv, BucketAr
 template <int LA, int LB, int LC, int LD>
 struct WaveCfg {
   using C = FastClass<LA, LB, LC, LD>;
-  static constexpr int ROWS = (C::NE + 63) / 64;
+  // lanes per quartet: the rows rounded up to 16, 32 or 64 (two rows per
+  // lane above 64); a wave holds 64 / LPT quartets
+  static constexpr int LPT = C::NE <= 16 ? 16 : (C::NE <= 32 ? 32 : 64);
+  static constexpr int TPW = 64 / LPT;
+  static constexpr int ROWS = (C::NE + LPT - 1) / LPT;
   static constexpr int VS = (C::E + 1) * (C::F + 1);
   static constexpr int PV = 3 * C::NR * VS;  // recurrence tables of one primitive quartet
-  // primitive quartets whose tables share the LDS at once (20 KB of tables:
-  // two waves per SIMD)
-  static constexpr int PB = 2560 / PV < 1 ? 1 : (2560 / PV > 64 ? 64 : 2560 / PV);
+  // primitive quartets per quartet whose tables share the LDS at once (20 KB
+  // of tables per wave: two waves per SIMD)
+  static constexpr int PB0 = 2560 / (TPW * PV);
+  static constexpr int PB = PB0 < 1 ? 1 : (PB0 > LPT ? LPT : PB0);
   // compiled: accumulators that leave room for two waves per SIMD
-  static constexpr bool OK = C::NE >= 16 && ROWS * C::NF <= 96 &&
+  static constexpr bool OK = C::NE >= 6 && ROWS * C::NF <= 96 &&
                              !EpiCfg<LA, LB, LC, LD, true>::XG;
-  // used by default: blocks of 480+ components, measured faster than the
-  // chunk kernels on every such c1 bucket ((dd|dd) 1.6 -> 1.0 ms, (fd|fd)
-  // 3.5 -> 1.8 ms, (fd|dp) 9.1 -> 6.7 ms); below, the walk outweighs the
-  // products and the chunk kernels win ((fp|ps) 2.9 -> 11.8 ms)
+  // used by default (without a tuned table): blocks of 480+ components,
+  // measured faster than the chunk kernels on every such c1 bucket ((dd|dd)
+  // 1.6 -> 1.0 ms, (fd|fd) 3.5 -> 1.8 ms, (fd|dp) 9.1 -> 6.7 ms)
   static constexpr bool DEFAULT = OK && C::NE * C::NF >= 480;
 };
 
@@ -1514,15 +1520,18 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
   using C = FastClass<LA, LB, LC, LD>;
   using W = WaveCfg<LA, LB, LC, LD>;
   constexpr int NE = C::NE, NF = C::NF, E = C::E, F = C::F, NR = C::NR;
-  constexpr int ROWS = W::ROWS, VS = W::VS, PV = W::PV, PB = W::PB;
+  constexpr int LPT = W::LPT, TPW = W::TPW, ROWS = W::ROWS, VS = W::VS, PV = W::PV, PB = W::PB;
   constexpr int NEF = NE * NF;
-  __shared__ double s_V[PB * PV];
-  __shared__ double s_u[PB * NR], s_w[PB * NR];
-  __shared__ WavePrim s_p[PB];
-  const int64_t t = A.t0 + blockIdx.x;
-  if (t >= A.t1) return;  // the whole workgroup
+  __shared__ double s_V[TPW][PB * PV];
+  __shared__ double s_u[TPW][PB * NR], s_w[TPW][PB * NR];
+  __shared__ WavePrim s_p[TPW][PB];
+  const int64_t t_first = A.t0 + (int64_t)blockIdx.x * TPW;
+  if (t_first >= A.t1) return;  // the whole workgroup
   const int lane = threadIdx.x;
-  const Task tk = A.tasks[t];
+  const int sub = lane / LPT, sl = lane - sub * LPT;  // quartet of the wave, lane in it
+  const int64_t t = t_first + sub;
+  const bool valid = t < A.t1;
+  const Task tk = A.tasks[valid ? t : t_first];
   const VShell& sa = Bv.vsh[tk.sh[0]];
   const VShell& sb = Bv.vsh[tk.sh[1]];
   const VShell& sc = Bv.vsh[tk.sh[2]];
@@ -1531,13 +1540,13 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
   int rx[ROWS], ry[ROWS], rz[ROWS];
   bool has[ROWS];
   sfor<ROWS>([&](auto S) {
-    constexpr int sl = decltype(S)::value;
-    const int e = lane + 64 * sl;
-    has[sl] = e < NE;
-    const int ee = has[sl] ? e : 0;
-    rx[sl] = cpx(LA, ee);
-    ry[sl] = cpy(LA, ee);
-    rz[sl] = cpz(LA, ee);
+    constexpr int r = decltype(S)::value;
+    const int e = sl + LPT * r;
+    has[r] = e < NE;
+    const int ee = has[r] ? e : 0;
+    rx[r] = cpx(LA, ee);
+    ry[r] = cpy(LA, ee);
+    rz[r] = cpz(LA, ee);
   });
   double acc[ROWS][NF];
   sfor<ROWS>([&](auto S) {
@@ -1562,146 +1571,152 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
                           (LB == 0 ? 0.282094791773878143 : LB == 1 ? 0.488602511902919921 : 1.0) *
                           (LC == 0 ? 0.282094791773878143 : LC == 1 ? 0.488602511902919921 : 1.0) *
                           (LD == 0 ? 0.282094791773878143 : LD == 1 ? 0.488602511902919921 : 1.0);
-  // pair emptiness (src/cint2e.c:96-109)
+  const unsigned long long own = (LPT == 64 ? ~0ull : ((1ull << LPT) - 1)) << (sub * LPT);
+  // pair emptiness (src/cint2e.c:96-109), per quartet of the wave
   int live_ij = 0, live_kl = 0;
-  for (int p = lane; p < nij; p += 64) live_ij |= (pij[p].cce0 - dij) < Ecut;
-  for (int p = lane; p < nkl; p += 64) live_kl |= (pkl[p].cce0 - dkl) < Ecut;
-  const bool live = __ballot(live_ij) != 0 && __ballot(live_kl) != 0;
+  for (int p = sl; p < nij; p += LPT) live_ij |= (pij[p].cce0 - dij) < Ecut;
+  for (int p = sl; p < nkl; p += LPT) live_kl |= (pkl[p].cce0 - dkl) < Ecut;
+  const bool live = valid && (__ballot(live_ij) & own) != 0 && (__ballot(live_kl) & own) != 0;
+  const int npq = live ? nij * nkl : 0;
+  // candidate rounds common to the wave's quartets (barriers stay uniform)
+  int maxpq = npq;
+  for (int off = LPT; off < 64; off <<= 1) maxpq = max(maxpq, __shfl_xor(maxpq, off));
   int any = 0;
-  if (live) {
-    for (int base = 0; base < nij * nkl; base += 64) {
-      // one candidate per lane, in libcint's order (ip fastest)
-      const int p = base + lane;
-      bool pass = false;
-      WavePrim wp;
-      if (p < nij * nkl) {
-        const PairPrim P = pij[pos_ij[p % nij]];
-        const PairPrim Q = pkl[pos_kl[p / nij]];
-        const double cce_kl = Q.cce0 - dkl, cce_ij = P.cce0 - dij;
-        if (!(cce_kl > Ecut) && !(cce_ij > Ecut - cce_kl)) {
-          any = 1;
-          pass = cce_ij < Ecut && cce_kl < Ecut;  // else the stored exp factor is 0
-        }
-        if (pass) {
-          wp.aij = P.aij;
-          wp.akl = Q.aij;
-          wp.a1 = P.aij * Q.aij;
-          wp.a0 = wp.a1 * rcp_pos(P.aij + Q.aij);
-          wp.pq[0] = P.px - Q.px;
-          wp.pq[1] = P.py - Q.py;
-          wp.pq[2] = P.pz - Q.pz;
-          wp.pa[0] = P.px - sa.r[0];
-          wp.pa[1] = P.py - sa.r[1];
-          wp.pa[2] = P.pz - sa.r[2];
-          wp.qc[0] = Q.px - sc.r[0];
-          wp.qc[1] = Q.py - sc.r[1];
-          wp.qc[2] = Q.pz - sc.r[2];
-          const double rr = wp.pq[0] * wp.pq[0] + wp.pq[1] * wp.pq[1] + wp.pq[2] * wp.pq[2];
-          wp.fac = sqrt_pos(wp.a0 * rcp_pos(wp.a1 * wp.a1 * wp.a1)) * cfac * P.kab * Q.kab;
-          wp.x = wp.a0 * rr;
-          wp.theta = 1.0;
-          if (omega > 0) {
-            wp.theta = omega * omega * rcp_pos(omega * omega + wp.a0);
-            wp.x *= wp.theta;
-            wp.fac *= sqrt_pos(wp.theta);
-          }
-        }
+  for (int base = 0; base < maxpq; base += LPT) {
+    // one candidate per lane, in libcint's order (ip fastest)
+    const int p = base + sl;
+    bool pass = false;
+    WavePrim wp;
+    if (p < npq) {
+      const PairPrim P = pij[pos_ij[p % nij]];
+      const PairPrim Q = pkl[pos_kl[p / nij]];
+      const double cce_kl = Q.cce0 - dkl, cce_ij = P.cce0 - dij;
+      if (!(cce_kl > Ecut) && !(cce_ij > Ecut - cce_kl)) {
+        any = 1;
+        pass = cce_ij < Ecut && cce_kl < Ecut;  // else the stored exp factor is 0
       }
-      const unsigned long long mask = __ballot(pass);
-      const int nsurv = __popcll(mask);
-      const int slot = __popcll(mask & ((1ull << lane) - 1));
-      for (int b0 = 0; b0 < nsurv; b0 += PB) {
-        const int np = nsurv - b0 < PB ? nsurv - b0 : PB;
-        if (pass && slot >= b0 && slot < b0 + PB) s_p[slot - b0] = wp;
-        __syncthreads();
-        // roots and weights of every (primitive quartet, root)
-        for (int it = lane; it < np * NR; it += 64) {
-          const int q = it / NR, r = it - q * NR;
-          double u, w;
-          rys_root_one(NR, r, s_p[q].x, u, w);
-          if (omega > 0) {
-            const double ut = u * s_p[q].theta;
-            u = ut * rcp_pos(u + 1.0 - ut);
-          }
-          s_u[it] = u;
-          s_w[it] = w * s_p[q].fac;
+      if (pass) {
+        wp.aij = P.aij;
+        wp.akl = Q.aij;
+        wp.a1 = P.aij * Q.aij;
+        wp.a0 = wp.a1 * rcp_pos(P.aij + Q.aij);
+        wp.pq[0] = P.px - Q.px;
+        wp.pq[1] = P.py - Q.py;
+        wp.pq[2] = P.pz - Q.pz;
+        wp.pa[0] = P.px - sa.r[0];
+        wp.pa[1] = P.py - sa.r[1];
+        wp.pa[2] = P.pz - sa.r[2];
+        wp.qc[0] = Q.px - sc.r[0];
+        wp.qc[1] = Q.py - sc.r[1];
+        wp.qc[2] = Q.pz - sc.r[2];
+        const double rr = wp.pq[0] * wp.pq[0] + wp.pq[1] * wp.pq[1] + wp.pq[2] * wp.pq[2];
+        wp.fac = sqrt_pos(wp.a0 * rcp_pos(wp.a1 * wp.a1 * wp.a1)) * cfac * P.kab * Q.kab;
+        wp.x = wp.a0 * rr;
+        wp.theta = 1.0;
+        if (omega > 0) {
+          wp.theta = omega * omega * rcp_pos(omega * omega + wp.a0);
+          wp.x *= wp.theta;
+          wp.fac *= sqrt_pos(wp.theta);
         }
-        __syncthreads();
-        // 2D recurrence of every (primitive quartet, direction, root)
-        for (int it = lane; it < np * 3 * NR; it += 64) {
-          const int q = it / (3 * NR), dr = it - q * 3 * NR, d = dr / NR, r = dr - d * NR;
-          const WavePrim& P = s_p[q];
-          const double u2 = P.a0 * s_u[q * NR + r];
-          const double tmp4 = 0.5 * rcp_pos(u2 * (P.aij + P.akl) + P.a1);
-          const double tmp5 = u2 * tmp4;
-          const double b00 = tmp5, b10 = tmp5 + tmp4 * P.akl, b01 = tmp5 + tmp4 * P.aij;
-          const double c00 = P.pa[d] - 2.0 * tmp5 * P.akl * P.pq[d];
-          const double c0p = P.qc[d] + 2.0 * tmp5 * P.aij * P.pq[d];
-          double* V = s_V + q * PV + dr * VS;
-          V[0] = d == 2 ? s_w[q * NR + r] : 1.0;
-          sfor<E>([&](auto EE) {
-            constexpr int e = decltype(EE)::value;
-            double v = c00 * V[e * (F + 1)];
-            if constexpr (e > 0) v += e * b10 * V[(e - 1) * (F + 1)];
-            V[(e + 1) * (F + 1)] = v;
-          });
-          sfor<F>([&](auto FF) {
-            constexpr int f = decltype(FF)::value;
-            sfor<E + 1>([&](auto EE) {
-              constexpr int e = decltype(EE)::value;
-              double v = c0p * V[e * (F + 1) + f];
-              if constexpr (f > 0) v += f * b01 * V[e * (F + 1) + f - 1];
-              if constexpr (e > 0) v += e * b00 * V[(e - 1) * (F + 1) + f];
-              V[e * (F + 1) + f + 1] = v;
-            });
-          });
-        }
-        __syncthreads();
-        // [e0|f0] products of this lane's rows
-        for (int q = 0; q < np; ++q) {
-          for (int r = 0; r < NR; ++r) {
-            const double* Vx = s_V + q * PV + (0 * NR + r) * VS;
-            const double* Vy = s_V + q * PV + (1 * NR + r) * VS;
-            const double* Vz = s_V + q * PV + (2 * NR + r) * VS;
-            sfor<ROWS>([&](auto S) {
-              constexpr int sl = decltype(S)::value;
-              if (has[sl]) {
-                double vx[F + 1], vy[F + 1], vz[F + 1];
-                sfor<F + 1>([&](auto K) {
-                  constexpr int k = decltype(K)::value;
-                  vx[k] = Vx[rx[sl] * (F + 1) + k];
-                  vy[k] = Vy[ry[sl] * (F + 1) + k];
-                  vz[k] = Vz[rz[sl] * (F + 1) + k];
-                });
-                sfor<NF>([&](auto J) {
-                  constexpr int j = decltype(J)::value;
-                  constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
-                  acc[sl][j] = fma(vx[fx] * vy[fy], vz[fz], acc[sl][j]);
-                });
-              }
-            });
-          }
-        }
-        __syncthreads();
       }
     }
+    const unsigned long long mask = __ballot(pass) & own;
+    const int nsurv = __popcll(mask);
+    const int slot = __popcll(mask & ((1ull << lane) - 1));
+    int maxsurv = nsurv;
+    for (int off = LPT; off < 64; off <<= 1) maxsurv = max(maxsurv, __shfl_xor(maxsurv, off));
+    for (int b0 = 0; b0 < maxsurv; b0 += PB) {
+      const int np = nsurv - b0 < 0 ? 0 : (nsurv - b0 < PB ? nsurv - b0 : PB);
+      if (pass && slot >= b0 && slot < b0 + PB) s_p[sub][slot - b0] = wp;
+      __syncthreads();
+      // roots and weights of every (primitive quartet, root) of this quartet
+      for (int it = sl; it < np * NR; it += LPT) {
+        const int q = it / NR, r = it - q * NR;
+        double u, w;
+        rys_root_one(NR, r, s_p[sub][q].x, u, w);
+        if (omega > 0) {
+          const double ut = u * s_p[sub][q].theta;
+          u = ut * rcp_pos(u + 1.0 - ut);
+        }
+        s_u[sub][it] = u;
+        s_w[sub][it] = w * s_p[sub][q].fac;
+      }
+      __syncthreads();
+      // 2D recurrence of every (primitive quartet, direction, root)
+      for (int it = sl; it < np * 3 * NR; it += LPT) {
+        const int q = it / (3 * NR), dr = it - q * 3 * NR, d = dr / NR, r = dr - d * NR;
+        const WavePrim& P = s_p[sub][q];
+        const double u2 = P.a0 * s_u[sub][q * NR + r];
+        const double tmp4 = 0.5 * rcp_pos(u2 * (P.aij + P.akl) + P.a1);
+        const double tmp5 = u2 * tmp4;
+        const double b00 = tmp5, b10 = tmp5 + tmp4 * P.akl, b01 = tmp5 + tmp4 * P.aij;
+        const double c00 = P.pa[d] - 2.0 * tmp5 * P.akl * P.pq[d];
+        const double c0p = P.qc[d] + 2.0 * tmp5 * P.aij * P.pq[d];
+        double* V = s_V[sub] + q * PV + dr * VS;
+        V[0] = d == 2 ? s_w[sub][q * NR + r] : 1.0;
+        sfor<E>([&](auto EE) {
+          constexpr int e = decltype(EE)::value;
+          double v = c00 * V[e * (F + 1)];
+          if constexpr (e > 0) v += e * b10 * V[(e - 1) * (F + 1)];
+          V[(e + 1) * (F + 1)] = v;
+        });
+        sfor<F>([&](auto FF) {
+          constexpr int f = decltype(FF)::value;
+          sfor<E + 1>([&](auto EE) {
+            constexpr int e = decltype(EE)::value;
+            double v = c0p * V[e * (F + 1) + f];
+            if constexpr (f > 0) v += f * b01 * V[e * (F + 1) + f - 1];
+            if constexpr (e > 0) v += e * b00 * V[(e - 1) * (F + 1) + f];
+            V[e * (F + 1) + f + 1] = v;
+          });
+        });
+      }
+      __syncthreads();
+      // [e0|f0] products of this lane's rows
+      for (int q = 0; q < np; ++q) {
+        for (int r = 0; r < NR; ++r) {
+          const double* Vx = s_V[sub] + q * PV + (0 * NR + r) * VS;
+          const double* Vy = s_V[sub] + q * PV + (1 * NR + r) * VS;
+          const double* Vz = s_V[sub] + q * PV + (2 * NR + r) * VS;
+          sfor<ROWS>([&](auto S) {
+            constexpr int rw = decltype(S)::value;
+            if (has[rw]) {
+              double vx[F + 1], vy[F + 1], vz[F + 1];
+              sfor<F + 1>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                vx[k] = Vx[rx[rw] * (F + 1) + k];
+                vy[k] = Vy[ry[rw] * (F + 1) + k];
+                vz[k] = Vz[rz[rw] * (F + 1) + k];
+              });
+              sfor<NF>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                constexpr int fx = cpx(LC, j), fy = cpy(LC, j), fz = cpz(LC, j);
+                acc[rw][j] = fma(vx[fx] * vy[fy], vz[fz], acc[rw][j]);
+              });
+            }
+          });
+        }
+      }
+      __syncthreads();
+    }
   }
-  any = __ballot(any) != 0 && live;
+  any = (__ballot(any) & own) != 0 && live;
+  if (!valid) return;
   // column-major ([f][e]): for each column the lanes' rows are contiguous
-  const int64_t slot = t - A.t0;
-  double* __restrict__ dst = A.scratch + slot * NEF;
+  const int64_t slot_t = t - A.t0;
+  double* __restrict__ dst = A.scratch + slot_t * NEF;
   sfor<ROWS>([&](auto S) {
-    constexpr int sl = decltype(S)::value;
-    if (has[sl]) {
-      const int e = lane + 64 * sl;
+    constexpr int rw = decltype(S)::value;
+    if (has[rw]) {
+      const int e = sl + LPT * rw;
       sfor<NF>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        dst[j * NE + e] = acc[sl][j];
+        dst[j * NE + e] = acc[rw][j];
       });
     }
   });
-  if (lane == 0) {
-    A.flags[slot] = any;
+  if (sl == 0) {
+    A.flags[slot_t] = any;
     if (A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
   }
 }
@@ -1737,8 +1752,9 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
         (g_wave_kernels == 2 ||
          (g_wave_kernels == 1 && (cw ? cw == 2 : WaveCfg<LA, LB, LC, LD>::DEFAULT)))) {
       A.col_major = 1;
-      hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)n), dim3(64), 0, stream,
-                         B, A);
+      constexpr int TPW = WaveCfg<LA, LB, LC, LD>::TPW;
+      hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + TPW - 1) / TPW)),
+                         dim3(64), 0, stream, B, A);
       e = hipGetLastError();
     }
   }

@@ -109,10 +109,10 @@ def test_stratified_c1_int2e_sph(mc, orc):
 
 
 def test_wave_kernels_match_chunk_kernels_c1(mc):
-    """The team-of-one buckets of c1's chunked classes run on the wave
-    kernel (one wave per quartet); up to 20,000 quartets of each such bucket
-    give the same blocks with the component-chunk kernels (the other path of
-    the same buckets), to the parity bar of the oracle comparisons
+    """Every wave kernel (16, 32 or 64 lanes per quartet) against the
+    component-chunk kernels of the same buckets: up to 20,000 quartets of
+    each team-of-one bucket of c1's chunked classes give the same blocks on
+    both paths, to the parity bar of the oracle comparisons
     (1e-12 + 1e-10 |x|; the two paths sum primitives in different orders),
     plain and long-range operator."""
     import torch
@@ -137,7 +137,7 @@ def test_wave_kernels_match_chunk_kernels_c1(mc):
         dbo = mc.DeviceBasis(mc.Basis(m.atm, m.bas, env))
         pl = mc.Plan(dbo, "int2e_sph", sub)
         outs = []
-        for wave in (1, 0):
+        for wave in (2, 0):
             old = mc.tune(mc.TUNE_WAVE_KERNELS, wave)
             try:
                 o = torch.full((pl.out_size,), float("nan"), dtype=torch.float64, device="cuda")
