@@ -1502,6 +1502,10 @@ struct WaveCfg {
   // compiled: accumulators that leave room for two waves per SIMD
   static constexpr bool OK = C::NE >= 6 && ROWS * C::NF <= 96 &&
                              !EpiCfg<LA, LB, LC, LD, true>::XG;
+  // the epilogue in the same kernel (spherical store): the block and the
+  // bra result of every quartet of the wave in LDS, at most 32 KB
+  static constexpr int PERF = C::NE * C::NF + C::NAo * C::NBo * C::NF;
+  static constexpr bool FUSE = TPW * PERF <= 4096;
   // used by default (without a tuned table): blocks of 480+ components,
   // measured faster than the chunk kernels on every such c1 bucket ((dd|dd)
   // 1.6 -> 1.0 ms, (fd|fd) 3.5 -> 1.8 ms, (fd|dp) 9.1 -> 6.7 ms)
@@ -1522,13 +1526,20 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
   constexpr int NE = C::NE, NF = C::NF, E = C::E, F = C::F, NR = C::NR;
   constexpr int LPT = W::LPT, TPW = W::TPW, ROWS = W::ROWS, VS = W::VS, PV = W::PV, PB = W::PB;
   constexpr int NEF = NE * NF;
-  __shared__ double s_V[TPW][PB * PV];
-  __shared__ double s_u[TPW][PB * NR], s_w[TPW][PB * NR];
-  __shared__ WavePrim s_p[TPW][PB];
+  // LDS: the walk's tables (recurrence, roots, primitive data), then, when
+  // the epilogue is fused, each quartet's block and bra result
+  constexpr int WALK_D = TPW * (PB * PV + 2 * PB * NR + PB * (int)(sizeof(WavePrim) / 8));
+  constexpr int EPI_D = W::FUSE ? TPW * W::PERF : 0;
+  __shared__ double s_mem[WALK_D > EPI_D ? WALK_D : EPI_D];
   const int64_t t_first = A.t0 + (int64_t)blockIdx.x * TPW;
   if (t_first >= A.t1) return;  // the whole workgroup
   const int lane = threadIdx.x;
   const int sub = lane / LPT, sl = lane - sub * LPT;  // quartet of the wave, lane in it
+  double* const sV = s_mem + sub * (PB * PV);
+  double* const su = s_mem + TPW * PB * PV + sub * (PB * NR);
+  double* const sw = s_mem + TPW * PB * PV + TPW * PB * NR + sub * (PB * NR);
+  WavePrim* const sp =
+      reinterpret_cast<WavePrim*>(s_mem + TPW * (PB * PV + 2 * PB * NR)) + sub * PB;
   const int64_t t = t_first + sub;
   const bool valid = t < A.t1;
   const Task tk = A.tasks[valid ? t : t_first];
@@ -1627,33 +1638,33 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
     for (int off = LPT; off < 64; off <<= 1) maxsurv = max(maxsurv, __shfl_xor(maxsurv, off));
     for (int b0 = 0; b0 < maxsurv; b0 += PB) {
       const int np = nsurv - b0 < 0 ? 0 : (nsurv - b0 < PB ? nsurv - b0 : PB);
-      if (pass && slot >= b0 && slot < b0 + PB) s_p[sub][slot - b0] = wp;
+      if (pass && slot >= b0 && slot < b0 + PB) sp[slot - b0] = wp;
       __syncthreads();
       // roots and weights of every (primitive quartet, root) of this quartet
       for (int it = sl; it < np * NR; it += LPT) {
         const int q = it / NR, r = it - q * NR;
         double u, w;
-        rys_root_one(NR, r, s_p[sub][q].x, u, w);
+        rys_root_one(NR, r, sp[q].x, u, w);
         if (omega > 0) {
-          const double ut = u * s_p[sub][q].theta;
+          const double ut = u * sp[q].theta;
           u = ut * rcp_pos(u + 1.0 - ut);
         }
-        s_u[sub][it] = u;
-        s_w[sub][it] = w * s_p[sub][q].fac;
+        su[it] = u;
+        sw[it] = w * sp[q].fac;
       }
       __syncthreads();
       // 2D recurrence of every (primitive quartet, direction, root)
       for (int it = sl; it < np * 3 * NR; it += LPT) {
         const int q = it / (3 * NR), dr = it - q * 3 * NR, d = dr / NR, r = dr - d * NR;
-        const WavePrim& P = s_p[sub][q];
-        const double u2 = P.a0 * s_u[sub][q * NR + r];
+        const WavePrim& P = sp[q];
+        const double u2 = P.a0 * su[q * NR + r];
         const double tmp4 = 0.5 * rcp_pos(u2 * (P.aij + P.akl) + P.a1);
         const double tmp5 = u2 * tmp4;
         const double b00 = tmp5, b10 = tmp5 + tmp4 * P.akl, b01 = tmp5 + tmp4 * P.aij;
         const double c00 = P.pa[d] - 2.0 * tmp5 * P.akl * P.pq[d];
         const double c0p = P.qc[d] + 2.0 * tmp5 * P.aij * P.pq[d];
-        double* V = s_V[sub] + q * PV + dr * VS;
-        V[0] = d == 2 ? s_w[sub][q * NR + r] : 1.0;
+        double* V = sV + q * PV + dr * VS;
+        V[0] = d == 2 ? sw[q * NR + r] : 1.0;
         sfor<E>([&](auto EE) {
           constexpr int e = decltype(EE)::value;
           double v = c00 * V[e * (F + 1)];
@@ -1675,9 +1686,9 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
       // [e0|f0] products of this lane's rows
       for (int q = 0; q < np; ++q) {
         for (int r = 0; r < NR; ++r) {
-          const double* Vx = s_V[sub] + q * PV + (0 * NR + r) * VS;
-          const double* Vy = s_V[sub] + q * PV + (1 * NR + r) * VS;
-          const double* Vz = s_V[sub] + q * PV + (2 * NR + r) * VS;
+          const double* Vx = sV + q * PV + (0 * NR + r) * VS;
+          const double* Vy = sV + q * PV + (1 * NR + r) * VS;
+          const double* Vz = sV + q * PV + (2 * NR + r) * VS;
           sfor<ROWS>([&](auto S) {
             constexpr int rw = decltype(S)::value;
             if (has[rw]) {
@@ -1701,6 +1712,35 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
     }
   }
   any = (__ballot(any) & own) != 0 && live;
+  if constexpr (W::FUSE) {
+    if (A.sph && !A.jk_j && !A.jk_k) {
+      // epilogue here: the block to LDS (flat order), the bra transfer and
+      // spherical transform, then the ket and the store, on this quartet's
+      // lanes (the walk's LDS is free after its last barrier)
+      double* X = s_mem + sub * W::PERF;
+      double* Ys = X + NEF;
+      if (valid) {
+        sfor<ROWS>([&](auto S) {
+          constexpr int rw = decltype(S)::value;
+          if (has[rw]) {
+            const int e = sl + LPT * rw;
+            sfor<NF>([&](auto J) {
+              constexpr int j = decltype(J)::value;
+              X[e * NF + j] = acc[rw][j];
+            });
+          }
+        });
+      }
+      __syncthreads();
+      if (valid && any) epi_bra<LA, LB, LC, LD, true>(Bv, tk, X, Ys, sl, LPT, nullptr);
+      __syncthreads();
+      if (valid) {
+        epi_ket_store<LA, LB, LC, LD, true>(Bv, tk, Ys, A.out, any, sl, LPT);
+        if (sl == 0 && A.not0) A.not0[A.task_id ? A.task_id[t] : t] = any;
+      }
+      return;
+    }
+  }
   if (!valid) return;
   // column-major ([f][e]): for each column the lanes' rows are contiguous
   const int64_t slot_t = t - A.t0;
@@ -1761,6 +1801,10 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
   if (!A.col_major)
     ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
+  if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK && WaveCfg<LA, LB, LC, LD>::FUSE) {
+    // the wave kernel stored the blocks itself
+    if (A.col_major && A.sph && !A.jk_j && !A.jk_k) return hipSuccess;
+  }
   if constexpr (DER == 0) {
     if (A.jk_j || A.jk_k) {
       using Q = EpiJkCfg<LA, LB, LC, LD>;
