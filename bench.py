@@ -421,8 +421,8 @@ def main():
                     help="run only the buckets of these classes, e.g. '3221:1,3321:1' "
                          "(la lb lc ld : team), --steps times each, then exit (for counter runs)")
     ap.add_argument("--wave-kernels", type=int, default=1, choices=(0, 1, 2),
-                    help="team-of-one chunked buckets on the wave kernel: 0 never, 1 the "
-                         "classes measured faster (default), 2 every class that has one")
+                    help="chunked buckets on the wave kernel: 0 never, 1 where measured "
+                         "faster (default: the tuned table), 2 every bucket that has one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()

@@ -403,10 +403,11 @@ int mcint_schwarz(mcint_basis *b, int nshell, double *q);
  * 4): mcint_plan_execute on a stream captures the plan's buckets into that
  * many concurrent graph branches (buckets balanced by estimated work, one
  * scratch region per branch); takes effect at the next capture.
- * MCINT_TUNE_WAVE_KERNELS (default 1): team-of-one buckets of the chunked
- * classes run the wave kernel (one 64-lane wave per quartet) instead of
- * their component-chunk kernels: 0 never, 1 for the classes measured
- * faster, 2 for every class that has one; read at launch.
+ * MCINT_TUNE_WAVE_KERNELS (default 1): buckets of the chunked classes run
+ * the wave kernel (16-64 lanes per quartet) instead of their
+ * component-chunk kernels: 0 never, 1 where measured faster (the tuned
+ * table, mcint_tune_class_wave), 2 in every bucket of every class that has
+ * one; read at launch.
  * Applies to plans created afterwards; returns the previous value, or a
  * negative status. */
 #define MCINT_TUNE_PRIMS_PER_LANE 0
@@ -420,14 +421,15 @@ int mcint_tune(int key, int value);
  * restores the global value.  Written by the autotuner
  * (libcint_amd/autotune.py).  Returns the previous override. */
 int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int prims_per_lane);
-/* Per-class kernel choice for the team-of-one buckets of a plain int2e
- * class (canonical order): mode 2 the wave kernel (one 64-lane wave per
- * quartet; only where mcint_wave_kernel_exists), 1 the component-chunk
- * kernels, 0 the built-in rule (wave kernel for blocks of 480+ [e0|f0]
- * components).  Applies while MCINT_TUNE_WAVE_KERNELS is 1; read at launch.
- * Returns the previous mode or a negative status.  (No libcint
- * counterpart: the autotuner's second dimension.) */
-int mcint_tune_class_wave(int la, int lb, int lc, int ld, int mode);
+/* Kernel choice for the buckets of a chunked plain int2e class (canonical
+ * order) with teams of `team` lanes (1, 2, 4, .., 64): mode 2 the wave
+ * kernel (16-64 lanes per quartet; only where mcint_wave_kernel_exists), 1
+ * the component-chunk kernels, 0 the built-in rule (the wave kernel for
+ * team-of-one buckets of blocks of 480+ [e0|f0] components).  Applies while
+ * MCINT_TUNE_WAVE_KERNELS is 1; read at launch.  Returns the previous mode
+ * or a negative status.  (No libcint counterpart: the autotuner's second
+ * dimension.) */
+int mcint_tune_class_wave(int la, int lb, int lc, int ld, int team, int mode);
 int mcint_wave_kernel_exists(int la, int lb, int lc, int ld);
 
 const char *mcint_last_error(void);

@@ -124,7 +124,7 @@ def lib():
     L.mcint_tune_class.restype = ctypes.c_int
     L.mcint_tune_class.argtypes = [ctypes.c_int] * 6
     L.mcint_tune_class_wave.restype = ctypes.c_int
-    L.mcint_tune_class_wave.argtypes = [ctypes.c_int] * 5
+    L.mcint_tune_class_wave.argtypes = [ctypes.c_int] * 6
     L.mcint_wave_kernel_exists.restype = ctypes.c_int
     L.mcint_wave_kernel_exists.argtypes = [ctypes.c_int] * 4
     L.mcint_last_error.restype = ctypes.c_char_p
@@ -174,10 +174,11 @@ def tune_class(name: str, cls, prims_per_lane: int) -> int:
     return rc
 
 
-def tune_class_wave(cls, mode: int) -> int:
-    """Team-of-one buckets of a plain int2e class: 2 the wave kernel, 1 the
-    chunk kernels, 0 the built-in rule (mcint_tune_class_wave)."""
-    rc = lib().mcint_tune_class_wave(*[int(c) for c in cls], int(mode))
+def tune_class_wave(cls, mode: int, team: int = 1) -> int:
+    """Buckets of a chunked plain int2e class with teams of `team` lanes:
+    2 the wave kernel, 1 the chunk kernels, 0 the built-in rule
+    (mcint_tune_class_wave)."""
+    rc = lib().mcint_tune_class_wave(*[int(c) for c in cls], int(team), int(mode))
     if rc < 0:
         raise McintError(f"mcint_tune_class_wave failed ({rc}): {last_error()}")
     return rc

@@ -11,11 +11,11 @@ it is measured: for each candidate the plan is rebuilt and every bucket is
 timed alone with HIP events, the times are summed per class, and each class
 keeps its fastest candidate.
 
-Second dimension (plain int2e): a chunked class's team-of-one bucket runs
-either its component-chunk kernels or the wave kernel (one wave per
-quartet, ``mcint_tune_class_wave``); both are timed per bucket and the class
-keeps the faster one (``"wave"`` in the table: 2 wave kernel, 1 chunk
-kernels).
+Second dimension (plain int2e): a chunked class's bucket of each team size
+runs either its component-chunk kernels or the wave kernel (16-64 lanes per
+quartet, ``mcint_tune_class_wave``); both are timed per bucket and each
+(class, team size) keeps the faster one (``"wave"`` in the table, keyed
+"la,lb,lc,ld:team": 2 wave kernel, 1 chunk kernels).
 
     python -m libcint_amd.autotune out.json           # c1 / c3 / c4 workloads
     python -m libcint_amd.autotune --wave-only out.json   # kernel choice only,
@@ -96,18 +96,18 @@ def autotune(dbasis: DeviceBasis, name: str, quartets, candidates=CANDIDATES, re
 
 def autotune_wave(dbasis: DeviceBasis, quartets, reps: int = 2, margin: float = 0.03,
                   log=print) -> dict:
-    """Kernel of every chunked int2e_sph class's team-of-one bucket: the
-    wave kernel where it is at least `margin` faster than the chunk kernels,
-    else the chunk kernels ({"la,lb,lc,ld": 2 or 1})."""
+    """Kernel of every bucket of the chunked int2e_sph classes: the wave
+    kernel where it is at least `margin` faster than the chunk kernels, else
+    the chunk kernels ({"la,lb,lc,ld:team": 2 or 1})."""
     import torch
     plan = Plan(dbasis, "int2e_sph", quartets)
     out = torch.empty(max(plan.out_size, 1), dtype=torch.float64, device="cuda")
     one = [(i, b) for i, b in enumerate(plan.buckets())
-           if b["team"] == 1 and b["path"] == "chunked" and wave_kernel_exists(b["cls"])]
+           if b["path"] == "chunked" and wave_kernel_exists(b["cls"])]
     times = {}
     for mode in (1, 2):
         for _, b in one:
-            tune_class_wave(b["cls"], mode)
+            tune_class_wave(b["cls"], mode, b["team"])
         plan.execute(out.data_ptr(), graph=False)
         torch.cuda.synchronize()
         for i, b in one:
@@ -120,14 +120,14 @@ def autotune_wave(dbasis: DeviceBasis, quartets, reps: int = 2, margin: float = 
                 e1.record()
                 e1.synchronize()
                 best = min(best, e0.elapsed_time(e1) * 1e-3)
-            times.setdefault(tuple(b["cls"]), {})[mode] = best
+            times.setdefault((tuple(b["cls"]), b["team"]), {})[mode] = best
     plan.close()
     table = {}
-    for cls, by in sorted(times.items()):
+    for (cls, team), by in sorted(times.items()):
         choice = 2 if by[2] < (1.0 - margin) * by[1] else 1
-        tune_class_wave(cls, 0)
-        table[",".join(str(c) for c in cls)] = choice
-        log(f"[autotune] wave {cls}: chunk kernels {by[1] * 1e3:.3f} ms, wave kernel "
+        tune_class_wave(cls, 0, team)
+        table[",".join(str(c) for c in cls) + ":%d" % team] = choice
+        log(f"[autotune] wave {cls} team {team}: chunk kernels {by[1] * 1e3:.3f} ms, wave kernel "
             f"{by[2] * 1e3:.3f} ms -> {'wave' if choice == 2 else 'chunk'}")
     return table
 
@@ -143,10 +143,11 @@ def apply(tables: dict) -> None:
     under "wave", {"la,lb,lc,ld": 2 wave kernel | 1 chunk kernels}."""
     for name, table in tables.items():
         if name == "wave":
-            for cls, mode in table.items():
+            for key, mode in table.items():
+                cls, _, team = key.partition(":")
                 c = [int(v) for v in cls.split(",")]
                 if int(mode) != 2 or wave_kernel_exists(c):
-                    tune_class_wave(c, int(mode))
+                    tune_class_wave(c, int(mode), int(team or 1))
             continue
         if name not in INTOR_CODES:
             continue

@@ -572,9 +572,11 @@ extern "C" int mcint_tune_class(int intor, int la, int lb, int lc, int ld, int p
   return old;
 }
 
-extern "C" int mcint_tune_class_wave(int la, int lb, int lc, int ld, int mode) {
+extern "C" int mcint_tune_class_wave(int la, int lb, int lc, int ld, int team, int mode) {
+  int tl = 0;
+  while ((1 << tl) < team && tl < 7) ++tl;
   if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > LMAX || lb > LMAX || lc > LMAX || ld > LMAX ||
-      mode < 0 || mode > 2) {
+      team < 1 || team > 64 || (1 << tl) != team || mode < 0 || mode > 2) {
     set_err("mcint_tune_class_wave: bad argument");
     return MCINT_ERR_ARG;
   }
@@ -582,7 +584,7 @@ extern "C" int mcint_tune_class_wave(int la, int lb, int lc, int ld, int mode) {
     set_err("mcint_tune_class_wave: the class has no wave kernel");
     return MCINT_ERR_UNSUPPORTED;
   }
-  int8_t& v = g_class_wave[class_index(la, lb, lc, ld)];
+  int8_t& v = g_class_wave[class_index(la, lb, lc, ld) * 7 + tl];
   const int old = v;
   v = (int8_t)mode;
   return old;

@@ -807,7 +807,7 @@ hipError_t launch_generic(const BasisView& B, const BucketArgs& A, hipStream_t s
 
 namespace mcint {
 int g_wave_kernels = 1;
-int8_t g_class_wave[NCLS] = {};
+int8_t g_class_wave[NCLS * 7] = {};
 #define MC_WAVE_HAS(a, b, c, d) \
   if (la == a && lb == b && lc == c && ld == d) return WaveCfg<a, b, c, d>::OK ? 1 : 0;
 int wave_kernel_exists(int la, int lb, int lc, int ld) {

@@ -1787,10 +1787,15 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
   if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK) {
     // team-of-one buckets (few primitive quartets per quartet), plain
     // operator and long range, FP64
-    const int cw = g_class_wave[class_index(LA, LB, LC, LD)];
-    if (A.team == 1 && B.omega >= 0 && !A.fp32 &&
+    // per class and team size: the tuned choice, else the built-in rule
+    // (team-of-one buckets, blocks of 480+ components)
+    int tl = 0;
+    while ((1 << tl) < A.team && tl < 6) ++tl;
+    const int cw = g_class_wave[class_index(LA, LB, LC, LD) * 7 + tl];
+    const bool dflt = A.team == 1 && WaveCfg<LA, LB, LC, LD>::DEFAULT;
+    if (B.omega >= 0 && !A.fp32 &&
         (g_wave_kernels == 2 ||
-         (g_wave_kernels == 1 && (cw ? cw == 2 : WaveCfg<LA, LB, LC, LD>::DEFAULT)))) {
+         (g_wave_kernels == 1 && (cw ? cw == 2 : dflt)))) {
       A.col_major = 1;
       constexpr int TPW = WaveCfg<LA, LB, LC, LD>::TPW;
       hipLaunchKernelGGL((eri_wave_kernel<LA, LB, LC, LD>), dim3((unsigned)((n + TPW - 1) / TPW)),

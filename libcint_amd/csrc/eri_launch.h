@@ -202,13 +202,14 @@ int fast_kernel_mode(int kind, int sph, int la, int lb, int lc, int ld, double o
 // instead of their chunk kernels (mcint_tune MCINT_TUNE_WAVE_KERNELS;
 // read at launch, so graphs captured earlier keep their kernels)
 extern int g_wave_kernels;
-// per-class choice for the team-of-one buckets (mcint_tune_class_wave):
-// 0 the default rule, 1 chunk kernels, 2 wave kernel; indexed by class_index
+// per-class and team-size choice of the chunked classes' kernels
+// (mcint_tune_class_wave): 0 the default rule, 1 chunk kernels, 2 wave
+// kernel; indexed by class_index * 7 + log2(team lanes)
 constexpr int NCLS = (LMAX + 1) * (LMAX + 1) * (LMAX + 1) * (LMAX + 1);
 __host__ __device__ constexpr int class_index(int la, int lb, int lc, int ld) {
   return ((la * (LMAX + 1) + lb) * (LMAX + 1) + lc) * (LMAX + 1) + ld;
 }
-extern int8_t g_class_wave[NCLS];
+extern int8_t g_class_wave[NCLS * 7];
 // 1 when the canonical plain class (la lb|lc ld) has a wave kernel
 int wave_kernel_exists(int la, int lb, int lc, int ld);
 int chunked_block_doubles(int kind, int la, int lb, int lc, int ld);

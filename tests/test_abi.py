@@ -86,7 +86,9 @@ def test_wave_kernel_choice_is_validated_on_the_host(libpath):
     assert not mc.wave_kernel_exists((3, 3, 3, 3))   # 2 x 74 accumulators per lane: no
     with pytest.raises(mc.McintError):
         mc.tune_class_wave((1, 0, 0, 0), 2)
-    old = mc.tune_class_wave((3, 2, 2, 1), 2)
-    assert mc.tune_class_wave((3, 2, 2, 1), old) == 2
+    old = mc.tune_class_wave((3, 2, 2, 1), 2, team=8)
+    assert mc.tune_class_wave((3, 2, 2, 1), old, team=8) == 2
     with pytest.raises(mc.McintError):
         mc.tune_class_wave((3, 2, 2, 1), 3)
+    with pytest.raises(mc.McintError):
+        mc.tune_class_wave((3, 2, 2, 1), 2, team=3)

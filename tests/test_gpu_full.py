@@ -111,8 +111,9 @@ def test_stratified_c1_int2e_sph(mc, orc):
 def test_wave_kernels_match_chunk_kernels_c1(mc):
     """Every wave kernel (16, 32 or 64 lanes per quartet) against the
     component-chunk kernels of the same buckets: up to 20,000 quartets of
-    each team-of-one bucket of c1's chunked classes give the same blocks on
-    both paths, to the parity bar of the oracle comparisons
+    each bucket (every team size: one to several candidate rounds and
+    batches of primitive quartets) of c1's chunked classes give the same
+    blocks on both paths, to the parity bar of the oracle comparisons
     (1e-12 + 1e-10 |x|; the two paths sum primitives in different orders),
     plain and long-range operator."""
     import torch
@@ -125,7 +126,7 @@ def test_wave_kernels_match_chunk_kernels_c1(mc):
     rng = np.random.default_rng(5)
     picks = []
     for i, b in enumerate(plan.buckets()):
-        if b["team"] == 1 and b["path"] == "chunked":
+        if b["path"] == "chunked" and mc.wave_kernel_exists(b["cls"]):
             ids = plan.bucket_tasks(i, 0, int(b["tasks"])).astype(np.int64)
             picks.append(np.sort(rng.choice(ids, size=min(20000, ids.size), replace=False)))
     plan.close()
