@@ -1506,6 +1506,9 @@ struct WaveCfg {
   // bra result of every quartet of the wave in LDS, at most 32 KB
   static constexpr int PERF = C::NE * C::NF + C::NAo * C::NBo * C::NF;
   static constexpr bool FUSE = TPW * PERF <= 4096;
+  // the J/K digest in the same kernel: also the six density-block sums
+  static constexpr int PERJ = PERF + EpiJkCfg<LA, LB, LC, LD>::MJK;
+  static constexpr bool FUSE_JK = TPW * PERJ <= 4096;
   // used by default (without a tuned table): blocks of 480+ components,
   // measured faster than the chunk kernels on every such c1 bucket ((dd|dd)
   // 1.6 -> 1.0 ms, (fd|fd) 3.5 -> 1.8 ms, (fd|dp) 9.1 -> 6.7 ms)
@@ -1529,7 +1532,7 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
   // LDS: the walk's tables (recurrence, roots, primitive data), then, when
   // the epilogue is fused, each quartet's block and bra result
   constexpr int WALK_D = TPW * (PB * PV + 2 * PB * NR + PB * (int)(sizeof(WavePrim) / 8));
-  constexpr int EPI_D = W::FUSE ? TPW * W::PERF : 0;
+  constexpr int EPI_D = W::FUSE_JK ? TPW * W::PERJ : (W::FUSE ? TPW * W::PERF : 0);
   __shared__ double s_mem[WALK_D > EPI_D ? WALK_D : EPI_D];
   const int64_t t_first = A.t0 + (int64_t)blockIdx.x * TPW;
   if (t_first >= A.t1) return;  // the whole workgroup
@@ -1712,6 +1715,54 @@ __global__ void __launch_bounds__(64) eri_wave_kernel(BasisView Bv, BucketArgs A
     }
   }
   any = (__ballot(any) & own) != 0 && live;
+  if constexpr (W::FUSE_JK) {
+    if (A.sph && (A.jk_j || A.jk_k)) {
+      // the J/K digest here: block and bra result in LDS as for the store,
+      // then the ket rows' products with the density summed into the six
+      // block updates in LDS (epi_ket_jk) and added once per entry
+      using Q = EpiJkCfg<LA, LB, LC, LD>;
+      double* X = s_mem + sub * W::PERJ;
+      double* Ys = X + NEF;
+      double* G = X + W::PERF;
+      if (valid) {
+        sfor<ROWS>([&](auto S) {
+          constexpr int rw = decltype(S)::value;
+          if (has[rw]) {
+            const int e = sl + LPT * rw;
+            sfor<NF>([&](auto J) {
+              constexpr int j = decltype(J)::value;
+              X[e * NF + j] = acc[rw][j];
+            });
+          }
+        });
+        for (int k = sl; k < Q::MJK; k += LPT) G[k] = 0.0;
+      }
+      __syncthreads();
+      if (valid && any) epi_bra<LA, LB, LC, LD, true>(Bv, tk, X, Ys, sl, LPT, nullptr);
+      __syncthreads();
+      if (valid && any) epi_ket_jk<LA, LB, LC, LD>(Bv, tk, Ys, G, A, sl, LPT);
+      __syncthreads();
+      if (valid && any) {
+        const double f = jk_degeneracy(sa.shell, sb.shell, sc.shell, sd.shell);
+        const int64_t N = A.nao;
+        const int64_t P0 = sa.ao, Q0 = sb.ao, R0 = sc.ao, S0 = sd.ao;
+        for (int m = sl; m < Q::MJK; m += LPT) {
+          const double v = G[m];
+          if (v == 0.0) continue;
+          const int r = m < Q::O1 ? 0 : m < Q::O2 ? 1 : m < Q::O3 ? 2 : m < Q::O4 ? 3 : m < Q::O5 ? 4 : 5;
+          double* dst = r < 2 ? A.jk_j : A.jk_k;
+          if (!dst) continue;
+          const int o0 = r == 0 ? 0 : r == 1 ? Q::O1 : r == 2 ? Q::O2 : r == 3 ? Q::O3 : r == 4 ? Q::O4 : Q::O5;
+          const int ncol = r == 0 ? Q::NBo : (r == 1 || r == 3 || r == 5) ? Q::NDo : Q::NCo;
+          const int64_t row0 = r == 0 ? P0 : r == 1 ? R0 : r <= 3 ? P0 : Q0;
+          const int64_t col0 = r == 0 ? Q0 : r == 1 ? S0 : (r == 2 || r == 4) ? R0 : S0;
+          const int i = (m - o0) / ncol, j = (m - o0) % ncol;
+          atomicAdd(dst + (row0 + i) * N + col0 + j, (r < 2 ? 2.0 * f : f) * v);
+        }
+      }
+      return;
+    }
+  }
   if constexpr (W::FUSE) {
     if (A.sph && !A.jk_j && !A.jk_k) {
       // epilogue here: the block to LDS (flat order), the bra transfer and
@@ -1806,9 +1857,11 @@ hipError_t launch_chunks(const BasisView& B, const BucketArgs& A0, hipStream_t s
   if (!A.col_major)
     ((e = (e == hipSuccess ? launch_one_chunk<LA, LB, LC, LD, CHs, DER>(B, A, stream, grid) : e)), ...);
   if (e != hipSuccess) return e;
-  if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK && WaveCfg<LA, LB, LC, LD>::FUSE) {
-    // the wave kernel stored the blocks itself
-    if (A.col_major && A.sph && !A.jk_j && !A.jk_k) return hipSuccess;
+  if constexpr (DER == 0 && WaveCfg<LA, LB, LC, LD>::OK) {
+    // the wave kernel stored or digested the blocks itself
+    using W = WaveCfg<LA, LB, LC, LD>;
+    const bool jk = A.jk_j || A.jk_k;
+    if (A.col_major && A.sph && (jk ? W::FUSE_JK : W::FUSE)) return hipSuccess;
   }
   if constexpr (DER == 0) {
     if (A.jk_j || A.jk_k) {

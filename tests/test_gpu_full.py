@@ -157,6 +157,41 @@ def test_wave_kernels_match_chunk_kernels_c1(mc):
         assert (err <= 1e-12 + 1e-10 * np.abs(outs[1])).all(), float(err.max())
 
 
+def test_fused_jk_through_wave_kernels_c1(mc):
+    """J/K of c1 quartets from every chunked bucket (up to 5,000 each, every
+    team size) with the digest fused into the kernels, the wave kernel taken
+    wherever it exists, equals the stored-buffer digest of the same list."""
+    from libcint_amd import jk
+    from libcint_amd.synthetic import make_molecule, random_density, screened_quartets
+    m = make_molecule("c1")
+    B = mc.Basis(m.atm, m.bas, m.env)
+    db = mc.DeviceBasis(B)
+    q, _ = screened_quartets(db.schwarz(m.nbas), m.nbas, 1e-12)
+    plan = mc.Plan(db, "int2e_sph", q)
+    rng = np.random.default_rng(9)
+    picks = []
+    for i, b in enumerate(plan.buckets()):
+        if b["path"] == "chunked":
+            ids = plan.bucket_tasks(i, 0, int(b["tasks"])).astype(np.int64)
+            picks.append(rng.choice(ids, size=min(5000, ids.size), replace=False))
+    plan.close()
+    db.close()
+    sub = q[np.unique(np.concatenate(picks))]
+    dm = random_density(m.nao_sph(), seed=21)
+    old = mc.tune(mc.TUNE_WAVE_KERNELS, 2)
+    try:
+        vj_f, vk_f = jk.get_jk(B, dm, quartets=sub)
+    finally:
+        mc.tune(mc.TUNE_WAVE_KERNELS, old)
+    vj_s, vk_s = jk.get_jk(B, dm, quartets=sub, fused=False)
+    dj = float((vj_f - vj_s).abs().max())
+    dk = float((vk_f - vk_s).abs().max())
+    _say("c1 fused J/K through the wave kernels vs stored digest:", len(sub), "quartets,",
+         f"max|dJ| {dj:.2e} max|dK| {dk:.2e}")
+    assert dj < 1e-12 * max(1.0, float(vj_s.abs().max()))
+    assert dk < 1e-12 * max(1.0, float(vk_s.abs().max()))
+
+
 def test_stratified_c3_ip1_and_cart(mc, orc):
     from libcint_amd.jk import gradient_quartets
     from libcint_amd.synthetic import make_molecule, unique_quartets
