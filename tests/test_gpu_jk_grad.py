@@ -102,6 +102,30 @@ def test_fused_jk_paths(c0_setup):
     np.testing.assert_allclose(vk.cpu().numpy(), K, rtol=0, atol=tol_k)
 
 
+def test_fused_jk_long_range_and_short_range_refusal():
+    """Long range (omega = 0.3): the fused J/K equals the stored-buffer digest
+    of the erf-attenuated integrals (c3); short range (omega < 0) has no fused
+    digest and is refused with a status, not computed wrongly."""
+    import libcint_amd as mc
+    from libcint_amd import jk
+    from libcint_amd.synthetic import make_molecule, random_density
+    m = make_molecule("c3")
+    dm = random_density(m.nao_sph(), seed=6)
+    env = m.env.copy()
+    env[8] = 0.3
+    B = mc.Basis(m.atm, m.bas, env)
+    vj_s, vk_s = jk.get_jk(B, dm, fused=False)
+    vj_f, vk_f = jk.get_jk(B, dm)
+    dj = float((vj_f - vj_s).abs().max())
+    dk = float((vk_f - vk_s).abs().max())
+    _say(f"c3 long range fused vs stored: max|dJ| {dj:.2e} max|dK| {dk:.2e}")
+    assert dj < 1e-12 * max(1.0, float(vj_s.abs().max()))
+    assert dk < 1e-12 * max(1.0, float(vk_s.abs().max()))
+    env[8] = -0.3
+    with pytest.raises(mc.McintError):
+        jk.get_jk(mc.Basis(m.atm, m.bas, env), dm)
+
+
 def test_fused_jk_matches_stored_on_c3():
     """c3 (20 atoms, d shells): the fused J/K equals the stored-buffer
     digest, in FP64 and with the FP32 tier (the same blocks, summed in
