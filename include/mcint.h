@@ -281,8 +281,8 @@ void mcint_basis_destroy(mcint_basis *b);
 
 /* Build a plan for a host list of shell tuples (stride 4): canonical class
  * buckets, primitive-pair data, launch table.  out_offsets (host, nullable
- * = packed) in doubles.  *out_size receives the number of doubles the
- * output must hold. */
+ * = packed) in doubles; a negative offset returns MCINT_ERR_ARG.
+ * *out_size receives the number of doubles the output must hold. */
 int mcint_plan_create(mcint_plan **p, mcint_basis *b, int intor, const FINT *shls, size_t nq,
                       const int64_t *out_offsets, int64_t *out_size);
 /* Mixed precision: as mcint_plan_create, plus fp32_flags[q] != 0 marks

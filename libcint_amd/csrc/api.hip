@@ -769,6 +769,15 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
   }
   if (out_offsets) {
     int64_t mx = 0;
+    for (size_t q = 0; q < nq; ++q)
+      if (out_offsets[q] < 0) {
+        char msg[128];
+        snprintf(msg, sizeof msg, "mcint_plan_create: quartet %lld has a negative output offset",
+                 (long long)q);
+        set_err(msg);
+        plan_free(p);
+        return MCINT_ERR_ARG;
+      }
     for (size_t q = 0; q < nq; ++q) mx = std::max(mx, out_offsets[q] + qoff[q + 1]);
     for (size_t q = 0; q < nq; ++q) qoff[q] = out_offsets[q];
     p->out_size = mx;
@@ -1449,6 +1458,11 @@ extern "C" int mcint_jk_plan_create(mcint_jk_plan** out, mcint_basis* b, const F
       blk *= ao[sq[a] + 1] - ao[sq[a]];
     }
     off[q] = out_offsets ? out_offsets[q] : run;
+    if (off[q] < 0) {
+      jk_plan_free(j);
+      set_err("mcint_jk_plan_create: negative block offset");
+      return MCINT_ERR_ARG;
+    }
     run += blk;
   }
   const size_t lds = ((size_t)2 * dmax * j->nao + 256) * sizeof(double);
