@@ -112,8 +112,10 @@ struct mcint_basis {
   int device = 0;
 };
 
-extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm, const FINT* bas,
-                                  FINT nbas, const double* env) {
+// upload = false stops after the host tables (no HIP call): the basis then
+// serves only the host stage of plan construction (plan_build_host_only)
+static int basis_create_impl(mcint_basis** out, const FINT* atm, FINT natm, const FINT* bas,
+                             FINT nbas, const double* env, bool upload) {
   if (!out || !atm || !bas || !env || natm <= 0 || nbas <= 0) {
     set_err("mcint_basis_create: null pointer or empty basis");
     return MCINT_ERR_ARG;
@@ -201,6 +203,10 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
     set_err("mcint_basis_create: more than 32767 contracted shell columns");
     return MCINT_ERR_UNSUPPORTED;
   }
+  if (!upload) {
+    *out = b;
+    return MCINT_OK;
+  }
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = dalloc(&b->vsh_d, b->vsh.size());
   if (e == hipSuccess) e = dalloc(&b->env_d, b->env.size());
@@ -221,6 +227,11 @@ extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm,
   }
   *out = b;
   return MCINT_OK;
+}
+
+extern "C" int mcint_basis_create(mcint_basis** out, const FINT* atm, FINT natm, const FINT* bas,
+                                  FINT nbas, const double* env) {
+  return basis_create_impl(out, atm, natm, bas, nbas, env, true);
 }
 
 extern "C" void mcint_basis_destroy(mcint_basis* b) {
@@ -637,9 +648,11 @@ extern "C" int mcint_tune(int key, int value) {
   return MCINT_ERR_ARG;
 }
 
+// upload = false returns the plan after its host stage (buckets, task list,
+// pair table) without any HIP call; such a plan must never be executed
 static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const FINT* shls,
                             size_t nq, const int64_t* out_offsets, const uint8_t* fp32_flags,
-                            int64_t* out_size) {
+                            int64_t* out_size, bool upload = true) {
   if (!out || !b || (!shls && nq)) {
     set_err("mcint_plan_create: null argument");
     return MCINT_ERR_ARG;
@@ -977,6 +990,12 @@ static int plan_create_impl(mcint_plan** out, mcint_basis* b, int intor, const F
       pairs.push_back(a);
       pairs.push_back(c);
     }
+  if (!upload) {
+    p->scratch_size = max_gen;
+    p->flags_size = max_flags;
+    *out = p;
+    return MCINT_OK;
+  }
   int* pairs_d = nullptr;
   PairPrim* pp_tmp = nullptr;
   hipError_t e = dalloc(&p->tasks_d, (size_t)std::max<int64_t>(ntasks, 1));
@@ -1671,6 +1690,31 @@ int mcint::plan_launch_scratch(mcint_plan* p, double* out, FINT* not0, hipStream
 }
 
 int64_t mcint::plan_tasks(const mcint_plan* p) { return p->ntasks; }
+
+int mcint::plan_build_host_only(const FINT* atm, FINT natm, const FINT* bas, FINT nbas,
+                                const double* env, int intor, const FINT* shls, size_t nq,
+                                const int64_t* out_offsets, const uint8_t* fp32_flags,
+                                int64_t* stats, std::vector<int64_t>* bucket_info) {
+  mcint_basis* b = nullptr;
+  int st = basis_create_impl(&b, atm, natm, bas, nbas, env, false);
+  if (st != MCINT_OK) return st;
+  mcint_plan* p = nullptr;
+  st = plan_create_impl(&p, b, intor, shls, nq, out_offsets, fp32_flags, nullptr, false);
+  if (st == MCINT_OK) {
+    if (stats) mcint_plan_stats(p, stats);
+    if (bucket_info) {
+      bucket_info->clear();
+      for (int i = 0; i < (int)p->buckets.size(); ++i) {
+        int64_t info[8];
+        mcint_plan_bucket_info(p, i, info);
+        bucket_info->insert(bucket_info->end(), info, info + 8);
+      }
+    }
+    plan_free(p);
+  }
+  mcint_basis_destroy(b);
+  return st;
+}
 
 void mcint::set_error(const char* msg) { set_err(msg); }
 

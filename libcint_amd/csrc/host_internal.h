@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/mcint.h"
 
 namespace mcint {
@@ -16,6 +18,19 @@ int plan_launch_scratch(mcint_plan* p, double* out, FINT* not0, hipStream_t stre
 
 // Tasks (contracted quartets after the virtual-shell expansion) of plan p.
 int64_t plan_tasks(const mcint_plan* p);
+
+// The host stage of plan construction alone, with no HIP call: the basis
+// tables and the plan's validation, canonical classes, buckets, counting
+// sort and pair table for the list, as mcint_basis_create followed by
+// mcint_plan_create[_mixed] would build them before the device upload.
+// Writes the statistics of mcint_plan_stats (6 entries, nullable) and the
+// 8 entries of mcint_plan_bucket_info per bucket (nullable), then frees
+// everything.  For host-side test builds (sanitizers) on machines without
+// a GPU.  Returns the status plan creation would return.
+int plan_build_host_only(const FINT* atm, FINT natm, const FINT* bas, FINT nbas, const double* env,
+                         int intor, const FINT* shls, size_t nq, const int64_t* out_offsets,
+                         const uint8_t* fp32_flags, int64_t* stats,
+                         std::vector<int64_t>* bucket_info);
 
 // Sets the calling thread's mcint_last_error() text.
 void set_error(const char* msg);
