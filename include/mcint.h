@@ -199,6 +199,16 @@ void CINTshells_cart_offset(FINT ao_loc[], const FINT *bas, const FINT nbas);
  * (libcint calls exit() instead). */
 void CINTrys_roots(int nroots, double x, double *u, double *w);
 int mcint_rys_roots_batch(int nroots, const double *x, size_t n, double *u, double *w);
+/* Short-range rule: roots u = t^2/(1-t^2) and weights of exp(-x t^2) on
+ * [lower, 1], evaluated on the device (replaces src/rys_roots.c:149-267,
+ * declared at src/rys_roots.h:4); nroots <= 9, 0 <= lower < 1.  Built by
+ * the discretised Stieltjes procedure on 1024 Gauss-Legendre samples of the
+ * weight and Sturm bisection of the Jacobi matrix.  The batch form takes
+ * host arrays of n (x, lower) pairs and fills n * nroots roots and weights.
+ * Errors as for CINTrys_roots. */
+void CINTsr_rys_roots(int nroots, double x, double lower, double *u, double *w);
+int mcint_sr_rys_roots_batch(int nroots, const double *x, const double *lower, size_t n,
+                             double *u, double *w);
 
 /* ---- 2. host-pointer batches -------------------------------------------
  * out + out_offsets[q] receives quartet q, byte-identical in layout to

@@ -142,6 +142,10 @@ def lib():
     L.CINTrys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
     L.mcint_rys_roots_batch.restype = ctypes.c_int
     L.mcint_rys_roots_batch.argtypes = [ctypes.c_int, dp, ctypes.c_size_t, dp, dp]
+    L.CINTsr_rys_roots.restype = None
+    L.CINTsr_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
+    L.mcint_sr_rys_roots_batch.restype = ctypes.c_int
+    L.mcint_sr_rys_roots_batch.argtypes = [ctypes.c_int, dp, dp, ctypes.c_size_t, dp, dp]
     L.CINTgto_norm.restype = ctypes.c_double
     L.CINTgto_norm.argtypes = [ctypes.c_int, ctypes.c_double]
     _lib = L
@@ -269,6 +273,20 @@ def rys_roots(nroots: int, x):
     w = np.zeros((x.size, nroots))
     _check(lib().mcint_rys_roots_batch(nroots, _dp(x), x.size, _dp(u), _dp(w)),
            "mcint_rys_roots_batch")
+    return u, w
+
+
+def sr_rys_roots(nroots: int, x, lower):
+    """Device short-range rule (weight exp(-x t^2) on [lower, 1]) for arrays of
+    x and lower (broadcast together): returns (u[n, nroots], w[n, nroots])."""
+    x, lower = np.broadcast_arrays(np.atleast_1d(np.asarray(x, dtype=np.float64)),
+                                   np.atleast_1d(np.asarray(lower, dtype=np.float64)))
+    x = np.ascontiguousarray(x.ravel())
+    lower = np.ascontiguousarray(lower.ravel())
+    u = np.zeros((x.size, nroots))
+    w = np.zeros((x.size, nroots))
+    _check(lib().mcint_sr_rys_roots_batch(nroots, _dp(x), _dp(lower), x.size, _dp(u), _dp(w)),
+           "mcint_sr_rys_roots_batch")
     return u, w
 
 

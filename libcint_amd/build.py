@@ -27,7 +27,7 @@ LIB = os.path.join(HERE, "libmcint.so")
 SOURCES = ([f"ip1_{g}.hip" for g in (14, *range(14))]
            + [f"chunk3_{g}.hip" for g in range(4)] + [f"fast3_{g}.hip" for g in range(4)]
            + [f"chunk_{g}.hip" for g in (17, 11, 18, 19, 20, 21, 12, 13, 14, 15, 16, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0)]
-           + [f"fast_{g}.hip" for g in (7, 3, 8, 5, 6, 4, 2, 1, 0)] + ["eri.hip", "jk.hip", "api.hip", "noalloc.hip"])
+           + [f"fast_{g}.hip" for g in (7, 3, 8, 5, 6, 4, 2, 1, 0)] + ["eri.hip", "jk.hip", "api.hip", "noalloc.hip", "sr_roots.hip"])
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCINT_ARCH", "gfx950")

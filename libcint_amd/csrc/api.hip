@@ -2378,6 +2378,50 @@ extern "C" void CINTrys_roots(int nroots, double x, double* u, double* w) {
   (void)mcint_rys_roots_batch(nroots, &x, 1, u, w);
 }
 
+extern "C" int mcint_sr_rys_roots_batch(int nroots, const double* x, const double* lower,
+                                        size_t n, double* u, double* w) {
+  if (nroots < 1 || nroots > NRYS_MAX || ((!x || !lower || !u || !w) && n)) {
+    set_err("mcint_sr_rys_roots_batch: nroots must be 1..9 and the arrays non-null");
+    return MCINT_ERR_ARG;
+  }
+  if (n == 0) return MCINT_OK;
+  if (n > (size_t)INT32_MAX) {
+    set_err("mcint_sr_rys_roots_batch: more than 2^31-1 rules in one call");
+    return MCINT_ERR_ARG;
+  }
+  for (size_t i = 0; i < n; ++i)
+    if (!(x[i] >= 0) || std::isinf(x[i]) || !(lower[i] >= 0) || !(lower[i] < 1)) {
+      set_err("mcint_sr_rys_roots_batch: x must be finite and >= 0, lower in [0, 1)");
+      return MCINT_ERR_ARG;
+    }
+  double *xd = nullptr, *ld = nullptr, *ud = nullptr, *wd = nullptr;
+  hipError_t e = dalloc(&xd, n);
+  if (e == hipSuccess) e = dalloc(&ld, n);
+  if (e == hipSuccess) e = dalloc(&ud, n * nroots);
+  if (e == hipSuccess) e = dalloc(&wd, n * nroots);
+  if (e == hipSuccess) e = hipMemcpy(xd, x, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ld, lower, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_sr_rys_batch(nroots, xd, ld, (int64_t)n, ud, wd, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(u, ud, n * nroots * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(w, wd, n * nroots * sizeof(double), hipMemcpyDeviceToHost);
+  dfree(xd);
+  dfree(ld);
+  dfree(ud);
+  dfree(wd);
+  if (e != hipSuccess) {
+    set_err(std::string("mcint_sr_rys_roots_batch: ") + hipGetErrorString(e));
+    return MCINT_ERR_HIP;
+  }
+  return MCINT_OK;
+}
+
+// libcint's signature (src/rys_roots.h:4) returns nothing; on an invalid
+// argument or a device error u and w are left untouched and
+// mcint_last_error() says why (libcint prints and calls exit()).
+extern "C" void CINTsr_rys_roots(int nroots, double x, double lower, double* u, double* w) {
+  (void)mcint_sr_rys_roots_batch(nroots, &x, &lower, 1, u, w);
+}
+
 // ---------------------------------------------------------------------------
 // Schwarz bounds
 // ---------------------------------------------------------------------------

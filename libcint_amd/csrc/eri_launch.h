@@ -169,6 +169,10 @@ hipError_t launch_pairdata(const VShell* vsh, const double* env, int nv, const i
 
 hipError_t launch_rys_batch(int n, const double* x, int64_t count, double* u, double* w,
                             hipStream_t stream);
+// Short-range rule of exp(-x t^2) on [lower, 1] for count (x, lower) pairs
+// (one wave each); u, w: count * n.
+hipError_t launch_sr_rys_batch(int n, const double* x, const double* lower, int64_t count,
+                               double* u, double* w, hipStream_t stream);
 
 hipError_t launch_count_prims(const BasisView& B, const BucketArgs& A, unsigned long long* counter,
                               hipStream_t stream);

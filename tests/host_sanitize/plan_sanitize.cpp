@@ -308,6 +308,15 @@ int main() {
         int3c2e_sph(buf.data(), nullptr, sh, atm, B.natm, bas, B.nbas, env, nullptr, nullptr);
       }
       CINTdel_optimizer(&opt);
+      // Rys rules: full range and short range, single and batched
+      double ur[9], wr[9];
+      CINTrys_roots(5, 3.0, ur, wr);
+      CINTsr_rys_roots(5, 3.0, 0.4, ur, wr);
+      const double xs[3] = {0.0, 2.5, 80.0}, lows[3] = {0.0, 0.3, 0.9};
+      std::vector<double> ub(27), wb(27);
+      check(mcint_sr_rys_roots_batch(9, xs, lows, 3, ub.data(), wb.data()), "sr roots batch");
+      expect(mcint_sr_rys_roots_batch(9, xs, xs, 3, ub.data(), wb.data()), MCINT_ERR_ARG,
+             "sr roots with lower >= 1");
       const size_t nb = std::min<size_t>(nq, 2000);
       std::vector<size_t> boff(nb);
       size_t tot = 0;

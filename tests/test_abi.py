@@ -33,7 +33,7 @@ def test_header_declares_reference_entry_points():
               "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "CINTdel_optimizer",
               "CINTdel_2e_optimizer", "cint2e_sph", "cint2e_cart", "cint2e_sph_optimizer",
               "cint2e_ip1_sph", "CINTgto_norm", "CINTcgto_spheric", "CINTcgto_cart",
-              "CINTtot_cgto_spheric", "CINTshells_spheric_offset", "CINTrys_roots",
+              "CINTtot_cgto_spheric", "CINTshells_spheric_offset", "CINTrys_roots", "CINTsr_rys_roots",
               "int2e_sph_batch", "int2e_ip1_sph_batch", "int3c2e_sph_batch", "int2e_ip2_sph",
               "int2e_ip2_optimizer", "int2e_ipip1_sph", "int2e_ipip1_cart", "int2e_ipip1_optimizer",
               "int2e_ip1ip2_sph", "int2e_ip1ip2_cart", "int2e_ip1ip2_optimizer", "int3c2e_ip1_sph",

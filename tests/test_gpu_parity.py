@@ -56,6 +56,54 @@ def test_device_rys_roots(mc, orc):
             np.testing.assert_allclose(w[i], w0, rtol=1e-13, atol=0, err_msg=f"w n={n} x={x}")
 
 
+def test_device_short_range_rule(mc, orc):
+    """CINTsr_rys_roots on the device (weight exp(-x t^2) on [lower, 1]) against
+    the oracle's rule (itself pinned by its moments against mpmath), against
+    the device full-range rule at lower = 0, and by its own moments
+    sum_i w_i s_i^k = int_lower^1 t^2k exp(-x t^2) dt, k < 2n (closed form in
+    mpmath).
+    Cases with x lower^2 > 600 are left out: their weights are below
+    1e-260 and lose relative precision in double."""
+    import mpmath as mp
+
+    xs = np.array([0.0, 1e-6, 0.3, 2.0, 17.0, 60.0, 250.0, 1000.0])
+    lows = np.array([0.0, 0.05, 0.3, 0.7, 0.95])
+    X, L = np.meshgrid(xs, lows, indexing="ij")
+    keep = X * L * L <= 600
+    X, L = X[keep], L[keep]
+    worst_u = worst_w = 0.0
+    for n in range(1, 10):
+        u, w = mc.sr_rys_roots(n, X, L)
+        for i in range(X.size):
+            u0, w0 = orc.sr_rys_roots(n, float(X[i]), float(L[i]))
+            # roots compared in s = t^2 = u / (1 + u), weights relative to the largest
+            s, s0 = u[i] / (1 + u[i]), u0 / (1 + u0)
+            eu = np.max(np.abs(s - s0))
+            ew = np.max(np.abs(w[i] - w0)) / np.max(np.abs(w0))
+            worst_u, worst_w = max(worst_u, eu), max(worst_w, ew)
+            assert eu < 1e-13 and ew < 1e-11, (n, X[i], L[i], eu, ew)
+        uf, wf = mc.rys_roots(n, xs)
+        u0, w0 = mc.sr_rys_roots(n, xs, 0.0)
+        np.testing.assert_allclose(u0, uf, rtol=1e-11, atol=0, err_msg=f"lower=0 u n={n}")
+        np.testing.assert_allclose(w0, wf, rtol=1e-11, atol=0, err_msg=f"lower=0 w n={n}")
+    _say(f"short-range rule vs oracle: max |ds| {worst_u:.2e}, max rel |dw| {worst_w:.2e}")
+    # moments in closed form: I_0 = sqrt(pi)/(2 sqrt x) (erfc(sqrt x lower) -
+    # erfc(sqrt x)), I_k = (lower^(2k-1) e^(-x lower^2) - e^(-x) + (2k-1) I_(k-1)) / 2x
+    mp.mp.dps = 60
+    for n, x, lo in [(9, 2.0, 0.3), (6, 60.0, 0.05), (9, 250.0, 0.7), (4, 0.3, 0.95),
+                     (9, 250.0, 0.0)]:
+        u, w = mc.sr_rys_roots(n, x, lo)
+        s = u[0] / (1 + u[0])
+        xm, lo_m = mp.mpf(x), mp.mpf(lo)
+        mom = mp.sqrt(mp.pi) / (2 * mp.sqrt(xm)) * (mp.erfc(mp.sqrt(xm) * lo_m) - mp.erfc(mp.sqrt(xm)))
+        for k in range(2 * n):
+            if k:
+                mom = (lo_m ** (2 * k - 1) * mp.exp(-xm * lo_m ** 2) - mp.exp(-xm)
+                       + (2 * k - 1) * mom) / (2 * xm)
+            got = float(np.sum(w[0] * s ** k))
+            assert abs(got - float(mom)) <= 1e-12 * float(mom), (n, x, lo, k, got, float(mom))
+
+
 @pytest.mark.parametrize("name,key", [("int2e_sph", "cint2e_sph"), ("int2e_cart", None),
                                       ("int2e_ip1_sph", "cint2e_ip1_sph"),
                                       ("int2e_ip1_cart", None), ("int2e_ip2_sph", None),
