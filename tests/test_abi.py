@@ -92,3 +92,28 @@ def test_wave_kernel_choice_is_validated_on_the_host(libpath):
         mc.tune_class_wave((3, 2, 2, 1), 3)
     with pytest.raises(mc.McintError):
         mc.tune_class_wave((3, 2, 2, 1), 2, team=3)
+
+
+def test_short_range_rule_arguments_are_checked_on_the_host(libpath):
+    """mcint_sr_rys_roots_batch and CINTsr_rys_roots refuse bad arguments
+    before any device call (nroots outside 1..9, x < 0 or not finite, lower
+    outside [0, 1)); CINTsr_rys_roots then leaves u and w untouched.  An
+    empty batch succeeds without touching the device."""
+    import ctypes
+
+    import numpy as np
+
+    import libcint_amd as mc
+    L = mc.lib()
+    for n, x, lo in [(0, 1.0, 0.2), (10, 1.0, 0.2), (3, -1.0, 0.2), (3, float("inf"), 0.2),
+                     (3, float("nan"), 0.2), (3, 1.0, 1.0), (3, 1.0, -0.1)]:
+        with pytest.raises(mc.McintError):
+            mc.sr_rys_roots(n, x, lo)
+        u = np.full(max(n, 1), 7.0)
+        w = np.full(max(n, 1), 7.0)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.CINTsr_rys_roots(n, x, lo, u.ctypes.data_as(dp), w.ctypes.data_as(dp))
+        assert (u == 7.0).all() and (w == 7.0).all()
+        assert b"mcint_sr_rys_roots_batch" in L.mcint_last_error()
+    u, w = mc.sr_rys_roots(4, np.zeros(0), np.zeros(0))
+    assert u.shape == (0, 4) and w.shape == (0, 4)
