@@ -26,6 +26,10 @@ SUPP = os.path.join(SAN_DIR, "lsan.supp")
 def _run(name: str, ok_line: str):
     exe = os.path.join(SAN_DIR, "_build", name)
     if not os.path.exists(exe):
+        import libcint_amd
+        if not os.path.exists(libcint_amd.LIB_PATH):
+            from libcint_amd import build
+            build.build()
         subprocess.run(["make", "-s", "-C", SAN_DIR], check=True)
     env = dict(os.environ)
     env["ASAN_OPTIONS"] = "detect_leaks=1:halt_on_error=1"
